@@ -1,0 +1,479 @@
+"""GPU element assembler with the reference's ReorderedElementAssemblerBase surface.
+
+Mirrors core/petsc/reordered_element_base.py:643-1783 (single device: lo = 0, hi = n_free,
+no MPI exchange): the constructor builds the permutation, the layout and the CSR pattern,
+then -- instead of JAX closures and a numpy scatter plan -- uploads the element data and
+an owner-computes patch schedule to the GPU and creates a ``libfne_hip`` plan.
+
+Callbacks keep their reference signatures:
+
+* ``energy_fn(vec) -> float``                 (:1579-1611)
+* ``gradient_fn(vec, g) -> None``             (:1613-1633)
+* ``assemble_hessian(u_owned) -> timings``    (:1635-1641, :1733-1775; writes ``self.A``)
+* ``update_dirichlet(u_0_new)``, ``create_vec``, ``cleanup``, ``setup_summary``,
+  ``memory_summary``, ``callback_summary``.
+
+Vectors are :class:`DeviceVec` (FP64 on the GPU); ``assemble_hessian`` also accepts a host
+array.  There is no CPU path: a missing or failing HIP library raises.
+"""
+
+from __future__ import annotations
+
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+from . import native
+from .layout import build_global_layout, select_permutation, validate_permutation
+from .linalg import KSP, DeviceCSR
+from .patches import PatchPlan, build_p1_plan
+from .vec import DeviceVec
+
+
+class _SelfComm:
+    """Stand-in for a one-rank communicator (the single-GPU case of MPI.COMM_WORLD)."""
+
+    rank = 0
+    size = 1
+
+    def Get_rank(self):
+        return 0
+
+    def Get_size(self):
+        return 1
+
+    def allreduce(self, v, op=None):
+        return v
+
+    def allgather(self, v):
+        return [v]
+
+
+class ReorderedElementAssemblerBase:
+    """Generic GPU assembler; subclasses provide the energy family and element data."""
+
+    distribution_strategy = "single_device"
+    block_size = 1
+    coordinate_key = "nodes"
+    dirichlet_key = "u_0"
+    local_elem_data_keys: tuple[str, ...] = ()
+    near_nullspace_key: str | None = None
+    fne_kind: int = 0
+
+    def __init__(
+        self,
+        params,
+        comm=None,
+        adjacency=None,
+        *,
+        ksp_rtol=1e-3,
+        ksp_type="cg",
+        pc_type="jacobi",
+        ksp_max_it=10000,
+        pc_options=None,
+        reorder_mode="block_morton",
+        local_hessian_mode="element",
+        use_near_nullspace=False,
+        perm_override=None,
+        distribution_strategy=None,
+        reuse_hessian_value_buffers=True,
+        assembly_backend="coo",
+        petsc_log_events=False,
+        memory_guard_total_gib=None,
+        hessian_precision="fp64",
+        rows_per_patch=256,
+        chunk=None,
+        device="cuda",
+    ):
+        if comm is not None and int(comm.Get_size()) != 1:
+            raise ValueError("the GPU assembler runs one rank per device; pass a one-rank communicator")
+        if local_hessian_mode != "element":
+            raise ValueError(f"Unsupported local_hessian_mode={local_hessian_mode!r} (element only)")
+        if assembly_backend not in {"coo", "coo_local", "blocked_local"}:
+            raise ValueError(f"Unsupported assembly backend {assembly_backend!r}")
+        if not torch.cuda.is_available():
+            raise RuntimeError("ReorderedElementAssemblerBase needs a GPU (torch.cuda is unavailable)")
+        native.lib()  # fail loudly before any set-up work
+        self.comm = comm if comm is not None else _SelfComm()
+        self.rank = 0
+        self.size = 1
+        self.params = params
+        self.device = torch.device(device)
+        self.reorder_mode = str(reorder_mode)
+        self.local_hessian_mode = str(local_hessian_mode)
+        self.use_near_nullspace = bool(use_near_nullspace)
+        self.assembly_backend = str(assembly_backend)
+        self.hessian_precision = str(hessian_precision)
+        self.memory_guard_total_gib = memory_guard_total_gib
+        self.iter_timings: list[dict] = []
+        self._hvp_eval_mode = "element_patch_lds"
+        self._setup_timings: dict[str, float] = {}
+        self._callback_stats = {
+            "energy": {"calls": 0, "kernel": 0.0, "total": 0.0},
+            "gradient": {"calls": 0, "kernel": 0.0, "total": 0.0},
+            "hessian": {"calls": 0, "hvp_compute": 0.0, "total": 0.0},
+        }
+        t_total = time.perf_counter()
+
+        freedofs = np.asarray(params["freedofs"], dtype=np.int64).ravel()
+        t0 = time.perf_counter()
+        if perm_override is None:
+            perm = select_permutation(
+                self.reorder_mode,
+                adjacency=adjacency,
+                coords_all=np.asarray(params[self.coordinate_key], dtype=np.float64),
+                freedofs=freedofs,
+                n_parts=1,
+                block_size=int(self.block_size),
+            )
+        else:
+            perm = validate_permutation(perm_override, freedofs.size)
+        self._setup_timings["permutation"] = time.perf_counter() - t0
+
+        t0 = time.perf_counter()
+        self.layout = build_global_layout(
+            params, adjacency, perm, block_size=int(self.block_size), dirichlet_key=self.dirichlet_key
+        )
+        self._setup_timings["global_layout"] = time.perf_counter() - t0
+        self.part = SimpleNamespace(
+            perm=self.layout.perm,
+            iperm=self.layout.iperm,
+            lo=0,
+            hi=self.layout.n_free,
+            n_free=self.layout.n_free,
+            n_owned=self.layout.n_free,
+        )
+        self.dirichlet_full = np.asarray(params[self.dirichlet_key], dtype=np.float64).ravel().copy()
+
+        t0 = time.perf_counter()
+        self._build_device_plan(rows_per_patch=int(rows_per_patch), chunk=chunk)
+        self._setup_timings["scatter_build"] = time.perf_counter() - t0
+
+        t0 = time.perf_counter()
+        self._f_owned = np.asarray(self._build_rhs_owned(), dtype=np.float64)
+        self._f_dev = (
+            torch.as_tensor(self._f_owned).to(self.device) if np.any(self._f_owned != 0.0) else None
+        )
+        self._setup_timings["rhs_build"] = time.perf_counter() - t0
+
+        t0 = time.perf_counter()
+        self.A = DeviceCSR(self.layout.indptr, self.layout.indices, self.layout.n_free, self.hessian_precision, self.device)
+        self.ksp = KSP(ksp_type=ksp_type, pc_type=pc_type, rtol=ksp_rtol, max_it=ksp_max_it)
+        self._setup_timings["matrix_create"] = time.perf_counter() - t0
+        self._energy_out = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._setup_timings["total"] = time.perf_counter() - t_total
+
+    # ------------------------------------------------------------------------------------
+    # problem hooks
+    # ------------------------------------------------------------------------------------
+    def _element_geometry(self) -> np.ndarray:
+        """(n_geo, n_elem) FP64 element data in the kernel's order."""
+        raise NotImplementedError
+
+    def _kernel_params(self) -> list[float]:
+        return []
+
+    def _quadrature_table(self) -> np.ndarray | None:
+        return None
+
+    def _plan_flags(self) -> int:
+        return 0
+
+    def _build_rhs_owned(self) -> np.ndarray:
+        return np.zeros(self.layout.n_free, dtype=np.float64)
+
+    def _lds_bytes_per_element(self) -> int:
+        """LDS staged per element in the patch kernel (P1: 6 Hessian + 3 gradient FP64)."""
+        return 72
+
+    def _build_patch_plan(self, elem_free, rows_per_patch: int, chunk) -> PatchPlan:
+        return build_p1_plan(
+            elem_free,
+            self.layout.indptr,
+            self.layout.indices,
+            self.layout.n_free,
+            rows_per_patch=rows_per_patch,
+            chunk=int(chunk or 512),
+        )
+
+    # ------------------------------------------------------------------------------------
+    # device plan
+    # ------------------------------------------------------------------------------------
+    def _t(self, a, dtype) -> torch.Tensor:
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=dtype)).to(self.device)
+
+    def _build_device_plan(self, rows_per_patch: int, chunk) -> None:
+        lay = self.layout
+        elems = np.asarray(self.params["elems"], dtype=np.int64)
+        elem_free = lay.total_to_free_reord[elems]
+        plan = self._build_patch_plan(elem_free, rows_per_patch, chunk)
+        # LDS budget of one workgroup: CSR slice (FP64) + staged element data per chunk
+        budget = 64 * 1024
+        per_elem = self._lds_bytes_per_element()
+        avail = budget - 8 * int(plan.max_patch_nnz)
+        if avail < 64 * per_elem:
+            raise ValueError("patch CSR slice too large for LDS; reduce rows_per_patch")
+        if plan.chunk * per_elem > avail:
+            plan.chunk = int(max(64, (avail // per_elem) // 64 * 64))
+        self.patch_plan = plan
+        order = plan.elem_order
+        n_free = lay.n_free
+        codes = np.where(elem_free >= 0, elem_free, n_free + elems)[order]
+        geo = self._element_geometry()[:, order]
+        self.n_elem = int(elems.shape[0])
+        self.n_loc = int(elems.shape[1])
+        self._dev = {
+            "edof": self._t(codes.T, np.int32),
+            "egeo": self._t(geo, np.float64),
+            "indptr": self._t(lay.indptr, np.int32),
+            "indices": self._t(lay.indices, np.int32),
+            "prow": self._t(plan.prow, np.int32),
+            "pelem_ptr": self._t(plan.pelem_ptr, np.int32),
+            "pelem": self._t(plan.pelem, np.int32),
+            "inc_ptr": self._t(plan.inc_ptr, np.int32),
+            "inc": self._t(plan.inc.view(np.int32), np.int32),
+        }
+        if plan.inc_aux is not None:
+            self._dev["inc_aux"] = self._t(plan.inc_aux, np.int32)
+        q = self._quadrature_table()
+        if q is not None:
+            self._dev["qdata"] = self._t(q, np.float64)
+        self.dvals = self._t(self.dirichlet_full, np.float64)
+
+        d = native.FnePlanDesc()
+        d.kind = int(self.fne_kind)
+        d.n_elem = self.n_elem
+        d.n_loc = self.n_loc
+        d.n_geo = int(geo.shape[0])
+        d.n_free = int(n_free)
+        d.n_total = int(self.dirichlet_full.size)
+        d.nnz = int(lay.nnz)
+        d.edof = native.ptr(self._dev["edof"])
+        d.egeo = native.ptr(self._dev["egeo"])
+        d.eweight = None
+        d.indptr = native.ptr(self._dev["indptr"])
+        d.indices = native.ptr(self._dev["indices"])
+        d.n_patch = int(plan.n_patch)
+        d.chunk = int(plan.chunk)
+        d.max_patch_nnz = int(plan.max_patch_nnz)
+        d.max_row_len = int(plan.max_row_len)
+        d.prow = native.ptr(self._dev["prow"])
+        d.pelem_ptr = native.ptr(self._dev["pelem_ptr"])
+        d.pelem = native.ptr(self._dev["pelem"])
+        d.inc_ptr = native.ptr(self._dev["inc_ptr"])
+        d.inc = native.ptr(self._dev["inc"])
+        d.inc_aux = native.ptr(self._dev.get("inc_aux"))
+        prm = list(self._kernel_params()) + [0.0] * 8
+        for i in range(8):
+            d.param[i] = float(prm[i])
+        d.qdata = native.ptr(self._dev.get("qdata"))
+        d.nq = int(0 if q is None else self._quadrature_points())
+        d.flags = int(self._plan_flags())
+        self._desc = d
+        import ctypes
+
+        handle = ctypes.c_void_p()
+        native.check(native.lib().fne_plan_create(ctypes.byref(d), ctypes.byref(handle)), "fne_plan_create")
+        self._plan = handle
+
+    def _quadrature_points(self) -> int:
+        return int(np.asarray(self.params["w"]).size)
+
+    def set_kernel_params(self, values) -> None:
+        import ctypes
+
+        arr = (ctypes.c_double * 8)(*(list(map(float, values)) + [0.0] * 8)[:8])
+        native.check(native.lib().fne_plan_set_params(self._plan, arr), "fne_plan_set_params")
+
+    # ------------------------------------------------------------------------------------
+    # vectors
+    # ------------------------------------------------------------------------------------
+    def create_vec(self, full_array_reordered=None) -> DeviceVec:
+        if full_array_reordered is None:
+            return DeviceVec.zeros(self.layout.n_free, self.device)
+        arr = np.asarray(full_array_reordered, dtype=np.float64)
+        return DeviceVec(torch.as_tensor(arr).to(self.device))
+
+    def _as_device(self, u) -> torch.Tensor:
+        if isinstance(u, DeviceVec):
+            return u.t
+        if isinstance(u, torch.Tensor):
+            return u.to(device=self.device, dtype=torch.float64)
+        return torch.as_tensor(np.asarray(u, dtype=np.float64)).to(self.device)
+
+    # ------------------------------------------------------------------------------------
+    # callbacks
+    # ------------------------------------------------------------------------------------
+    def energy_device(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        out = self._energy_out if out is None else out
+        native.check(
+            native.lib().fne_energy(
+                self._plan,
+                native.ptr(x),
+                native.ptr(self.dvals),
+                native.ptr(self._f_dev),
+                native.ptr(out),
+                native.stream_ptr(),
+            ),
+            "fne_energy",
+        )
+        return out
+
+    def energy_fn(self, vec) -> float:
+        t0 = time.perf_counter()
+        val = float(self.energy_device(self._as_device(vec)).item())
+        dt = time.perf_counter() - t0
+        self._record_callback("energy", kernel=dt, total=dt)
+        return val
+
+    def gradient_device(self, x: torch.Tensor, g: torch.Tensor) -> None:
+        native.check(
+            native.lib().fne_gradient(
+                self._plan,
+                native.ptr(x),
+                native.ptr(self.dvals),
+                native.ptr(self._f_dev),
+                native.ptr(g),
+                native.stream_ptr(),
+            ),
+            "fne_gradient",
+        )
+
+    def gradient_fn(self, vec, g) -> None:
+        t0 = time.perf_counter()
+        if isinstance(g, DeviceVec):
+            self.gradient_device(self._as_device(vec), g.t)
+        else:
+            gt = torch.empty(self.layout.n_free, dtype=torch.float64, device=self.device)
+            self.gradient_device(self._as_device(vec), gt)
+            g[:] = gt.cpu().numpy()
+        dt = time.perf_counter() - t0
+        self._record_callback("gradient", kernel=dt, total=dt)
+
+    def hessian_device(self, x: torch.Tensor, g: torch.Tensor | None = None) -> None:
+        native.check(
+            native.lib().fne_hessian(
+                self._plan,
+                native.ptr(x),
+                native.ptr(self.dvals),
+                native.ptr(self._f_dev),
+                native.ptr(g),
+                native.ptr(self.A.data),
+                self.A.fne_precision,
+                native.stream_ptr(),
+            ),
+            "fne_hessian",
+        )
+
+    def assemble_hessian(self, u_owned, variant=2):
+        del variant
+        return self.assemble_hessian_element(u_owned)
+
+    def assemble_hessian_element(self, u_owned):
+        t_total = time.perf_counter()
+        x = self._as_device(u_owned)
+        t0 = time.perf_counter()
+        self.hessian_device(x)
+        t_k = time.perf_counter() - t0
+        timings = {
+            "elem_hessian_compute": t_k,
+            "scatter": 0.0,
+            "coo_assembly": 0.0,
+            "allgatherv": 0.0,
+            "ghost_exchange": 0.0,
+            "build_v_local": 0.0,
+            "p2p_exchange": 0.0,
+            "hvp_compute": t_k,
+            "extraction": 0.0,
+            "n_hvps": 0,
+            "assembly_mode": "element_patch_lds",
+            "total": time.perf_counter() - t_total,
+        }
+        self.iter_timings.append(timings)
+        self._record_callback("hessian", hvp_compute=t_k, total=timings["total"])
+        return timings
+
+    def element_eval(self, u_free, want_energy=True, want_grad=True, want_hess=True):
+        """Per-element energy (n_elem,), gradient (n_elem, n_loc), Hessian (n_elem, n_loc, n_loc).
+
+        Elements are returned in the mesh's original order.
+        """
+        x = self._as_device(u_free)
+        ne, nl = self.n_elem, self.n_loc
+        e = torch.empty(ne, dtype=torch.float64, device=self.device) if want_energy else None
+        g = torch.empty(nl * ne, dtype=torch.float64, device=self.device) if want_grad else None
+        h = torch.empty(nl * nl * ne, dtype=torch.float64, device=self.device) if want_hess else None
+        native.check(
+            native.lib().fne_element_eval(
+                self._plan,
+                native.ptr(x),
+                native.ptr(self.dvals),
+                native.ptr(e),
+                native.ptr(g),
+                native.ptr(h),
+                native.stream_ptr(),
+            ),
+            "fne_element_eval",
+        )
+        inv = np.empty_like(self.patch_plan.elem_order)
+        inv[self.patch_plan.elem_order] = np.arange(inv.size)
+        inv_t = torch.as_tensor(inv).to(self.device)
+        out = []
+        if want_energy:
+            out.append(e[inv_t])
+        if want_grad:
+            out.append(g.view(nl, ne).t()[inv_t])
+        if want_hess:
+            out.append(h.view(nl, nl, ne).permute(2, 0, 1)[inv_t])
+        return tuple(out)
+
+    def update_dirichlet(self, u_0_new) -> None:
+        self.dirichlet_full = np.asarray(u_0_new, dtype=np.float64).ravel().copy()
+        self.dvals.copy_(torch.as_tensor(self.dirichlet_full))
+
+    # ------------------------------------------------------------------------------------
+    # bookkeeping (same bucket names as the reference)
+    # ------------------------------------------------------------------------------------
+    def _record_callback(self, phase: str, **timings) -> None:
+        stats = self._callback_stats[phase]
+        stats["calls"] = int(stats.get("calls", 0)) + 1
+        for k, v in timings.items():
+            stats[k] = float(stats.get(k, 0.0)) + float(v)
+
+    def setup_summary(self) -> dict[str, float]:
+        return {str(k): float(v) for k, v in self._setup_timings.items()}
+
+    def callback_summary(self) -> dict:
+        return {ph: dict(st) for ph, st in self._callback_stats.items()}
+
+    def memory_summary(self) -> dict:
+        dev_bytes = sum(int(t.numel() * t.element_size()) for t in self._dev.values())
+        csr_bytes = int(self.A.data.numel() * self.A.data.element_size()) + 4 * (self.A.nnz + self.A.n + 1)
+        return {
+            "device_plan_bytes": dev_bytes,
+            "csr_bytes": csr_bytes,
+            "owned_nnz": int(self.layout.nnz),
+            "local_elements": int(self.n_elem),
+            "patch_count": int(self.patch_plan.n_patch),
+            "patch_halo_factor": float(self.patch_plan.halo_factor),
+            "assembly_backend": "patch_lds",
+            "matrix_type": self.A.getType(),
+        }
+
+    def cleanup(self) -> None:
+        if getattr(self, "_plan", None) is not None and self._plan.value:
+            native.lib().fne_plan_destroy(self._plan)
+            self._plan = None
+        self.ksp.destroy()
+        self.A.destroy()
+        self._dev = {}
+
+    def __del__(self):
+        try:
+            if getattr(self, "_plan", None) is not None and self._plan.value:
+                native.lib().fne_plan_destroy(self._plan)
+                self._plan = None
+        except Exception:
+            pass

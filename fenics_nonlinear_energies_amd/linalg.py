@@ -1,0 +1,396 @@
+"""CSR matrix object and Krylov solvers on the GPU (the ``A`` / ``ksp`` pair of the assembler).
+
+* ``DeviceCSR.mult`` is the CSR SpMV hot loop (replaces PETSc ``MatMult``,
+  ``hessian_matvec_fn = A.mult`` in core/petsc/scalar_problem_driver.py:294).
+* ``KSP`` reproduces the PETSc KSP semantics the reference relies on -- zero initial guess,
+  left preconditioning, convergence on the preconditioned residual norm
+  ``||B r_k|| <= max(rtol*||B r_0||, abstol)``, divergence at ``dtol``, iteration cap -- for
+  ``cg`` (KSPCG), ``gmres`` (KSPGMRES, restart 30, modified Gram-Schmidt) and ``stcg``
+  (Steihaug-Toint CG with a radius, as set by KSPCGSetRadius in core/petsc/trust_ksp.py).
+  The preconditioner is point Jacobi (``jacobi``; zero diagonal entries are replaced by 1
+  as PETSc's PCJACOBI does) or ``none``.  PETSc's hypre/GAMG have no GPU counterpart here.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import native
+from .vec import DeviceVec, dot_device
+
+
+class DeviceCSR:
+    """Square CSR matrix on the GPU with a fixed pattern; values FP64 or FP32."""
+
+    def __init__(self, indptr, indices, n: int, precision: str = "fp64", device="cuda"):
+        self.n = int(n)
+        self.indptr = torch.as_tensor(np.asarray(indptr, dtype=np.int32)).to(device)
+        self.indices = torch.as_tensor(np.asarray(indices, dtype=np.int32)).to(device)
+        self.nnz = int(self.indices.numel())
+        self.set_precision(precision)
+        row_len = np.diff(np.asarray(indptr, dtype=np.int64))
+        avg = float(row_len.mean()) if row_len.size else 1.0
+        # lanes per row: smallest power of two covering the average row in one pass
+        lpr = 1
+        while lpr < 64 and lpr < avg:
+            lpr *= 2
+        self.lanes_per_row = lpr
+
+    def set_precision(self, precision: str) -> None:
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.data = torch.zeros(self.nnz, dtype=dtype, device=self.indices.device)
+
+    @property
+    def fne_precision(self) -> int:
+        return native.FNE_FP64 if self.precision == "fp64" else native.FNE_FP32
+
+    def getSize(self):
+        return (self.n, self.n)
+
+    def getType(self) -> str:
+        return "seqaijhip"
+
+    def mult(self, x: DeviceVec, y: DeviceVec) -> None:
+        self.mult_tensor(x.t, y.t)
+
+    def mult_tensor(self, x: torch.Tensor, y: torch.Tensor) -> None:
+        prec = self.fne_precision | (self.lanes_per_row << 8)
+        native.check(
+            native.lib().fne_spmv(
+                self.n,
+                native.ptr(self.indptr),
+                native.ptr(self.indices),
+                native.ptr(self.data),
+                prec,
+                native.ptr(x),
+                native.ptr(y),
+                native.stream_ptr(),
+            ),
+            "fne_spmv",
+        )
+
+    def diagonal(self) -> torch.Tensor:
+        d = torch.empty(self.n, dtype=torch.float64, device=self.indices.device)
+        native.check(
+            native.lib().fne_csr_diag(
+                self.n,
+                native.ptr(self.indptr),
+                native.ptr(self.indices),
+                native.ptr(self.data),
+                self.fne_precision,
+                native.ptr(d),
+                native.stream_ptr(),
+            ),
+            "fne_csr_diag",
+        )
+        return d
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        return sp.csr_matrix(
+            (self.data.double().cpu().numpy(), self.indices.cpu().numpy(), self.indptr.cpu().numpy()),
+            shape=(self.n, self.n),
+        )
+
+    def destroy(self) -> None:
+        self.data = None
+
+
+# KSP converged reasons (PETSc numbering)
+KSP_CONVERGED_RTOL = 2
+KSP_CONVERGED_ATOL = 3
+KSP_CONVERGED_CG_NEG_CURVE = 5
+KSP_CONVERGED_STEP_LENGTH = 6
+KSP_DIVERGED_ITS = -3
+KSP_DIVERGED_DTOL = -4
+KSP_DIVERGED_BREAKDOWN = -5
+KSP_DIVERGED_INDEFINITE_PC = -8
+KSP_DIVERGED_NANORINF = -9
+KSP_DIVERGED_INDEFINITE_MAT = -10
+
+
+class KSP:
+    """Krylov solver with the PETSc KSP call surface used by the reference drivers."""
+
+    def __init__(self, ksp_type="cg", pc_type="jacobi", rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, restart=30):
+        self.ksp_type = str(ksp_type)
+        self.pc_type = str(pc_type)
+        self.rtol = float(rtol)
+        self.atol = float(atol)
+        self.dtol = float(dtol)
+        self.max_it = int(max_it)
+        self.restart = int(restart)
+        self.radius = None
+        self.A: DeviceCSR | None = None
+        self._inv_diag: torch.Tensor | None = None
+        self.its = 0
+        self.reason = 0
+        self.rnorm = float("nan")
+        self.pc_setup_count = 0
+
+    # PETSc-style configuration -----------------------------------------------------------
+    def setType(self, t: str) -> None:
+        self.ksp_type = str(t)
+
+    def setTolerances(self, rtol=None, atol=None, divtol=None, max_it=None) -> None:
+        if rtol is not None:
+            self.rtol = float(rtol)
+        if atol is not None:
+            self.atol = float(atol)
+        if divtol is not None:
+            self.dtol = float(divtol)
+        if max_it is not None:
+            self.max_it = int(max_it)
+
+    def setOperators(self, A: DeviceCSR) -> None:
+        self.A = A
+
+    def setCGRadius(self, radius: float) -> None:
+        self.radius = float(radius)
+
+    def setUp(self) -> None:
+        """Preconditioner set-up (Jacobi: inverse diagonal, zero entries -> 1)."""
+        if self.A is None:
+            raise RuntimeError("KSP.setUp called before setOperators")
+        if self.pc_type == "jacobi":
+            d = self.A.diagonal()
+            d = torch.where(d == 0.0, torch.ones_like(d), d)
+            self._inv_diag = 1.0 / d
+        elif self.pc_type == "none":
+            self._inv_diag = None
+        else:
+            raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | none)")
+        self.pc_setup_count += 1
+
+    def getIterationNumber(self) -> int:
+        return int(self.its)
+
+    def getConvergedReason(self) -> int:
+        return int(self.reason)
+
+    def getResidualNorm(self) -> float:
+        return float(self.rnorm)
+
+    def destroy(self) -> None:
+        self.A = None
+        self._inv_diag = None
+
+    # -------------------------------------------------------------------------------------
+    def _pc(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        if self._inv_diag is None:
+            z.copy_(r)
+        else:
+            torch.mul(r, self._inv_diag, out=z)
+
+    def _dot(self, a: torch.Tensor, b: torch.Tensor) -> float:
+        return float(dot_device(a, b).item())
+
+    def _converged(self, it: int, rnorm: float, rnorm0: float) -> int:
+        if not math.isfinite(rnorm):
+            return KSP_DIVERGED_NANORINF
+        if rnorm <= max(self.rtol * rnorm0, self.atol):
+            return KSP_CONVERGED_RTOL if rnorm > self.atol else KSP_CONVERGED_ATOL
+        if it > 0 and rnorm >= self.dtol * rnorm0:
+            return KSP_DIVERGED_DTOL
+        return 0
+
+    def solve(self, b: DeviceVec, x: DeviceVec) -> None:
+        if self._inv_diag is None and self.pc_type == "jacobi":
+            self.setUp()
+        t = self.ksp_type.lower()
+        if t == "cg":
+            self._solve_cg(b.t, x.t)
+        elif t == "gmres":
+            self._solve_gmres(b.t, x.t)
+        elif t == "stcg":
+            self._solve_stcg(b.t, x.t)
+        else:
+            raise ValueError(f"unsupported ksp_type {self.ksp_type!r} (cg | gmres | stcg)")
+
+    # KSPCG, preconditioned-norm convergence --------------------------------------------
+    def _solve_cg(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        A = self.A
+        x.zero_()
+        r = b.clone()
+        z = torch.empty_like(b)
+        w = torch.empty_like(b)
+        self._pc(r, z)
+        dp = math.sqrt(max(self._dot(z, z), 0.0))
+        rnorm0 = dp
+        self.its = 0
+        self.rnorm = dp
+        self.reason = self._converged(0, dp, rnorm0)
+        if self.reason:
+            return
+        beta = self._dot(z, r)
+        if beta < 0.0:
+            self.reason = KSP_DIVERGED_INDEFINITE_PC
+            return
+        p = z.clone()
+        dpiold = None
+        betaold = beta
+        for i in range(self.max_it):
+            if i > 0:
+                p.mul_(beta / betaold).add_(z)
+            A.mult_tensor(p, w)
+            dpi = self._dot(p, w)
+            if dpi == 0.0 or (dpiold is not None and dpi * dpiold <= 0.0) or not math.isfinite(dpi):
+                self.reason = KSP_DIVERGED_INDEFINITE_MAT if math.isfinite(dpi) else KSP_DIVERGED_NANORINF
+                return
+            dpiold = dpi
+            a = beta / dpi
+            x.add_(p, alpha=a)
+            r.add_(w, alpha=-a)
+            self._pc(r, z)
+            dp = math.sqrt(max(self._dot(z, z), 0.0))
+            self.its = i + 1
+            self.rnorm = dp
+            self.reason = self._converged(i + 1, dp, rnorm0)
+            if self.reason:
+                return
+            betaold = beta
+            beta = self._dot(z, r)
+            if beta < 0.0:
+                self.reason = KSP_DIVERGED_INDEFINITE_PC
+                return
+        self.reason = KSP_DIVERGED_ITS
+
+    # KSPGMRES(restart), left preconditioning, modified Gram-Schmidt ----------------------
+    def _solve_gmres(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        A = self.A
+        m = max(1, self.restart)
+        n = b.numel()
+        x.zero_()
+        V = torch.empty((m + 1, n), dtype=torch.float64, device=b.device)
+        w = torch.empty_like(b)
+        tmp = torch.empty_like(b)
+        self.its = 0
+        rnorm0 = None
+        while True:
+            # preconditioned residual r = B (b - A x)
+            if self.its == 0:
+                tmp.copy_(b)
+            else:
+                A.mult_tensor(x, tmp)
+                torch.sub(b, tmp, out=tmp)
+            self._pc(tmp, V[0])
+            beta = math.sqrt(max(self._dot(V[0], V[0]), 0.0))
+            if rnorm0 is None:
+                rnorm0 = beta
+                self.rnorm = beta
+                self.reason = self._converged(0, beta, rnorm0)
+                if self.reason:
+                    return
+            if beta == 0.0:
+                self.reason = KSP_CONVERGED_ATOL
+                return
+            V[0].mul_(1.0 / beta)
+            H = np.zeros((m + 1, m), dtype=np.float64)
+            g = np.zeros(m + 1, dtype=np.float64)
+            g[0] = beta
+            cs = np.zeros(m, dtype=np.float64)
+            sn = np.zeros(m, dtype=np.float64)
+            k_used = 0
+            for k in range(m):
+                A.mult_tensor(V[k], tmp)
+                self._pc(tmp, w)
+                for j in range(k + 1):
+                    h = self._dot(w, V[j])
+                    H[j, k] = h
+                    w.add_(V[j], alpha=-h)
+                hn = math.sqrt(max(self._dot(w, w), 0.0))
+                H[k + 1, k] = hn
+                if hn > 0.0:
+                    torch.mul(w, 1.0 / hn, out=V[k + 1])
+                for j in range(k):
+                    t0 = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                    H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                    H[j, k] = t0
+                den = math.hypot(H[k, k], H[k + 1, k])
+                if den == 0.0:
+                    cs[k], sn[k] = 1.0, 0.0
+                else:
+                    cs[k], sn[k] = H[k, k] / den, H[k + 1, k] / den
+                H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+                H[k + 1, k] = 0.0
+                g[k + 1] = -sn[k] * g[k]
+                g[k] = cs[k] * g[k]
+                k_used = k + 1
+                self.its += 1
+                self.rnorm = abs(g[k + 1])
+                self.reason = self._converged(self.its, self.rnorm, rnorm0)
+                if self.reason or self.its >= self.max_it or hn == 0.0:
+                    break
+            # x += V_k y,  H_k y = g_k
+            y = np.zeros(k_used, dtype=np.float64)
+            for i in range(k_used - 1, -1, -1):
+                s = g[i] - float(np.dot(H[i, i + 1 : k_used], y[i + 1 : k_used]))
+                y[i] = s / H[i, i] if H[i, i] != 0.0 else 0.0
+            if k_used:
+                coeff = torch.as_tensor(y, dtype=torch.float64, device=b.device)
+                x.add_(coeff @ V[:k_used])
+            if self.reason:
+                return
+            if self.its >= self.max_it:
+                self.reason = KSP_DIVERGED_ITS
+                return
+
+    # Steihaug-Toint truncated CG inside radius (Euclidean norm on the unknown) ------------
+    def _solve_stcg(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        A = self.A
+        delta = float("inf") if self.radius is None else float(self.radius)
+        x.zero_()
+        r = b.clone()
+        z = torch.empty_like(b)
+        w = torch.empty_like(b)
+        self._pc(r, z)
+        dp = math.sqrt(max(self._dot(z, z), 0.0))
+        rnorm0 = dp
+        self.its = 0
+        self.rnorm = dp
+        self.reason = self._converged(0, dp, rnorm0)
+        if self.reason:
+            return
+        rz = self._dot(r, z)
+        p = z.clone()
+        xx = 0.0  # ||x||^2
+        for i in range(self.max_it):
+            A.mult_tensor(p, w)
+            kappa = self._dot(p, w)
+            xp = self._dot(x, p)
+            pp = self._dot(p, p)
+            if kappa <= 0.0 or not math.isfinite(kappa):
+                tau = (-xp + math.sqrt(max(xp * xp + pp * (delta * delta - xx), 0.0))) / pp
+                x.add_(p, alpha=tau)
+                self.its = i + 1
+                self.reason = KSP_CONVERGED_CG_NEG_CURVE
+                return
+            alpha = rz / kappa
+            xx_new = xx + 2.0 * alpha * xp + alpha * alpha * pp
+            if xx_new >= delta * delta:
+                tau = (-xp + math.sqrt(max(xp * xp + pp * (delta * delta - xx), 0.0))) / pp
+                x.add_(p, alpha=tau)
+                self.its = i + 1
+                self.reason = KSP_CONVERGED_STEP_LENGTH
+                return
+            x.add_(p, alpha=alpha)
+            xx = xx_new
+            r.add_(w, alpha=-alpha)
+            self._pc(r, z)
+            dp = math.sqrt(max(self._dot(z, z), 0.0))
+            self.its = i + 1
+            self.rnorm = dp
+            self.reason = self._converged(i + 1, dp, rnorm0)
+            if self.reason:
+                return
+            rz_new = self._dot(r, z)
+            p.mul_(rz_new / rz).add_(z)
+            rz = rz_new
+        self.reason = KSP_DIVERGED_ITS

@@ -1,0 +1,150 @@
+"""Owner-computes patch schedule for deterministic, atomic-free gradient/Hessian assembly.
+
+Replaces the reference's scatter plan (``_build_scatter_data``,
+core/petsc/reordered_element_base.py:1464-1534) whose ``np.add.at`` segmented sum is
+replaced on the GPU by an owner-computes reduction:
+
+* the free DOFs (CSR rows, reordered) are cut into patches of ``rows_per_patch``
+  consecutive rows -- one workgroup each, one row per thread;
+* a patch evaluates every element that touches one of its rows (owned elements plus a
+  halo, as the reference's overlap domains do per MPI rank, :509-597) into LDS;
+* each row thread then sums, in a fixed order, the contributions listed in its incidence
+  list -- so each CSR value and gradient entry is written by exactly one thread, once.
+
+Elements are renumbered so that the owned elements of a patch are contiguous (the device
+element arrays use the new order; ``elem_order[new] = old``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+P1_SLOT_BITS = 13
+P1_POS_BITS = 5
+P1_POS_NONE = (1 << P1_POS_BITS) - 1
+
+
+@dataclass
+class PatchPlan:
+    n_patch: int
+    rows_per_patch: int
+    chunk: int
+    prow: np.ndarray  # int32 [n_patch+1]
+    pelem_ptr: np.ndarray  # int32 [n_patch+1]
+    pelem: np.ndarray  # int32, new element ids
+    elem_order: np.ndarray  # int64 [n_elem], new -> old
+    inc_ptr: np.ndarray  # int32 [n_rows+1]
+    inc: np.ndarray  # uint32 items
+    inc_aux: np.ndarray | None
+    max_patch_nnz: int
+    max_row_len: int
+
+    @property
+    def halo_factor(self) -> float:
+        owned = int(np.count_nonzero(self.elem_order >= 0))
+        return float(self.pelem.size) / max(1, owned)
+
+
+def _patch_rows(n_rows: int, rows_per_patch: int) -> np.ndarray:
+    n_patch = max(1, -(-n_rows // rows_per_patch)) if n_rows else 0
+    prow = np.minimum(np.arange(n_patch + 1, dtype=np.int64) * rows_per_patch, n_rows)
+    return prow
+
+
+def _elements_by_patch(row_of: np.ndarray, rows_per_patch: int, n_patch: int):
+    """Owner patch per element, new element order, and the (patch, element) incidence pairs.
+
+    row_of: (n_elem, k) assembly rows touched per element (-1 = none).
+    """
+    n_elem = row_of.shape[0]
+    valid = row_of >= 0
+    masked = np.where(valid, row_of, np.iinfo(np.int64).max)
+    min_row = masked.min(axis=1) if n_elem else np.zeros(0, dtype=np.int64)
+    owner = np.where(min_row < np.iinfo(np.int64).max, min_row // rows_per_patch, n_patch)
+    elem_order = np.argsort(owner, kind="stable").astype(np.int64)
+    new_of_old = np.empty(n_elem, dtype=np.int64)
+    new_of_old[elem_order] = np.arange(n_elem, dtype=np.int64)
+    owner_new = owner[elem_order]
+    row_new = row_of[elem_order]
+    valid_new = row_new >= 0
+    e_idx = np.broadcast_to(np.arange(n_elem, dtype=np.int64)[:, None], row_new.shape)[valid_new]
+    p_idx = row_new[valid_new] // rows_per_patch
+    pair_key = np.unique(p_idx * np.int64(n_elem) + e_idx)
+    pp = pair_key // n_elem
+    pe = pair_key % n_elem
+    halo = (owner_new[pe] != pp).astype(np.int64)
+    order = np.lexsort((pe, halo, pp))
+    pelem = pe[order]
+    pelem_ptr = np.zeros(n_patch + 1, dtype=np.int64)
+    pelem_ptr[1:] = np.cumsum(np.bincount(pp, minlength=n_patch))
+    slot_sorted = np.empty(pair_key.size, dtype=np.int64)
+    slot_sorted[order] = np.arange(pair_key.size, dtype=np.int64) - pelem_ptr[pp[order]]
+    return elem_order, new_of_old, row_new, pelem, pelem_ptr, pair_key, slot_sorted
+
+
+def build_p1_plan(
+    elem_free: np.ndarray,
+    indptr: np.ndarray,
+    indices: np.ndarray,
+    n_free: int,
+    *,
+    rows_per_patch: int = 256,
+    chunk: int = 512,
+) -> PatchPlan:
+    """Patch plan for 3-DOF scalar elements (P1 triangles)."""
+    elem_free = np.asarray(elem_free, dtype=np.int64)
+    n_elem, n_loc = elem_free.shape
+    if n_loc != 3:
+        raise ValueError("build_p1_plan expects 3 DOFs per element")
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    prow = _patch_rows(n_free, rows_per_patch)
+    n_patch = prow.size - 1
+    elem_order, _, ef, pelem, pelem_ptr, pair_key, slot_sorted = _elements_by_patch(
+        elem_free, rows_per_patch, n_patch
+    )
+    counts = np.diff(pelem_ptr)
+    if counts.size and counts.max() >= (1 << P1_SLOT_BITS):
+        raise ValueError("too many elements in one patch for 13-bit slots; reduce rows_per_patch")
+    row_len = np.diff(indptr)
+    max_row_len = int(row_len.max()) if row_len.size else 0
+    if max_row_len >= P1_POS_NONE:
+        raise ValueError(f"CSR row of length {max_row_len} exceeds the P1 kernel limit {P1_POS_NONE - 1}")
+
+    # incidences (row r of element e at local position a)
+    e_i, a_i = np.nonzero(ef >= 0)
+    r_i = ef[e_i, a_i]
+    p_i = r_i // rows_per_patch
+    slot = slot_sorted[np.searchsorted(pair_key, p_i * np.int64(n_elem) + e_i)]
+    csr_keys = np.repeat(np.arange(n_free, dtype=np.int64), row_len) * np.int64(n_free) + indices
+    item = slot.astype(np.uint64) | (a_i.astype(np.uint64) << np.uint64(P1_SLOT_BITS))
+    for b in range(3):
+        c = ef[e_i, b]
+        pos = np.full(e_i.size, P1_POS_NONE, dtype=np.int64)
+        ok = c >= 0
+        k = np.searchsorted(csr_keys, r_i[ok] * np.int64(n_free) + c[ok])
+        if not np.array_equal(csr_keys[k], r_i[ok] * np.int64(n_free) + c[ok]):
+            raise RuntimeError("element coupling missing from the CSR pattern")
+        pos[ok] = k - indptr[r_i[ok]]
+        item |= pos.astype(np.uint64) << np.uint64(P1_SLOT_BITS + 2 + P1_POS_BITS * b)
+    order = np.lexsort((slot, r_i))
+    inc = item[order].astype(np.uint32)
+    inc_ptr = np.zeros(n_free + 1, dtype=np.int64)
+    inc_ptr[1:] = np.cumsum(np.bincount(r_i, minlength=n_free))
+    patch_nnz = indptr[prow[1:]] - indptr[prow[:-1]]
+    return PatchPlan(
+        n_patch=n_patch,
+        rows_per_patch=rows_per_patch,
+        chunk=int(chunk),
+        prow=prow.astype(np.int32),
+        pelem_ptr=pelem_ptr.astype(np.int32),
+        pelem=pelem.astype(np.int32),
+        elem_order=elem_order,
+        inc_ptr=inc_ptr.astype(np.int32),
+        inc=inc,
+        inc_aux=None,
+        max_patch_nnz=int(patch_nnz.max()) if patch_nnz.size else 0,
+        max_row_len=max_row_len,
+    )

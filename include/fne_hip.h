@@ -1,0 +1,159 @@
+/*
+ * fne_hip.h -- C ABI of the MI355X Newton-step assembly engine (libfne_hip.so).
+ *
+ * The reference (Beremi/fenics_nonlinear_energies) evaluates per-element energies,
+ * gradients and Hessians with JAX closures and scatters them into a PETSc COO/CSR
+ * matrix with numpy.  This library replaces that path with hand-written gfx950
+ * kernels.  Every function is plain C: device pointers and sizes in, int status out.
+ *
+ * Conventions (mirroring the reference's only native FFI, core/coloring/coloring_custom.py:41-115
+ * and core/petsc/trust_ksp.py:55-73):
+ *   - return 0 on success, a negative FNE_E* code on failure; fne_last_error() gives text;
+ *   - buffers are caller-owned device memory (hipMalloc / torch); the hot-path calls
+ *     allocate nothing; plan creation allocates a small reduction scratch once;
+ *   - numerical failure (NaN/inf in an energy) is NOT an error: it propagates to the
+ *     caller exactly as the reference's Newton loop expects (core/petsc/minimizers.py:280-291);
+ *   - all work is enqueued on the given HIP stream (NULL = default stream);
+ *   - nothing here is thread-safe for concurrent calls on one plan.
+ *
+ * Free-DOF numbering: free DOFs are numbered 0..n_free-1 in the reordered order
+ * (GlobalLayout.perm of core/petsc/reordered_element_base.py:24-39).  An element DOF
+ * entry edof[a*n_elem + e] holds the reordered free index when the DOF is free and
+ * n_free + t (t = total DOF index into the Dirichlet vector) when it is constrained,
+ * which replaces local_vec_from_full (reordered_element_base.py:224-235).
+ */
+#ifndef FNE_HIP_H
+#define FNE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* fne_stream_t; /* == hipStream_t */
+
+/* status codes */
+#define FNE_OK 0
+#define FNE_EINVAL (-1)
+#define FNE_EHIP (-2)
+#define FNE_ENOMEM (-3)
+#define FNE_EUNSUPPORTED (-4)
+
+/* energy families (rows A1-A4 of the hot path) */
+#define FNE_KIND_PLAPLACE 1       /* plaplace/jax_petsc/reordered_element_assembler.py:16-19 */
+#define FNE_KIND_GINZBURG_LANDAU 2 /* ginzburg_landau/jax_petsc/reordered_element_assembler.py:18-24 */
+#define FNE_KIND_NEOHOOKEAN 3     /* hyperelasticity/jax_petsc/parallel_hessian_dof.py:17-52 */
+#define FNE_KIND_MOHR_COULOMB 4   /* slope_stability/jax/jax_energy.py:45-108 */
+
+/* Hessian / SpMV value precision */
+#define FNE_FP64 0
+#define FNE_FP32 1
+
+/*
+ * Plan descriptor.  Every pointer is a DEVICE pointer owned by the caller and must stay
+ * valid until fne_plan_destroy.  Layouts are structure-of-arrays so that a wavefront
+ * reading element e..e+63 issues coalesced loads.
+ */
+typedef struct fne_plan_desc {
+  int32_t kind;      /* FNE_KIND_* */
+  int32_t n_elem;    /* elements in the energy set (all elements of the mesh) */
+  int32_t n_loc;     /* DOFs per element: 3 (P1 scalar), 12 (P1 tet, 3 comps), 30 (P4, 2 comps) */
+  int32_t n_geo;     /* geometry doubles per element */
+  int32_t n_free;    /* free DOFs = CSR rows = CSR cols */
+  int32_t n_total;   /* length of the Dirichlet value vector */
+  int64_t nnz;       /* CSR nonzeros */
+  const int32_t* edof;    /* [n_loc][n_elem] element DOF map (see header comment) */
+  const double* egeo;     /* [n_geo][n_elem] element geometry, FP64 */
+  const double* eweight;  /* [n_elem] energy weights or NULL (all 1) */
+  const int32_t* indptr;  /* [n_free+1] CSR row pointer */
+  const int32_t* indices; /* [nnz] CSR column indices, ascending per row */
+  /* patch schedule for the owner-computes gradient/Hessian assembly */
+  int32_t n_patch;        /* number of row patches (one workgroup each) */
+  int32_t chunk;          /* elements staged in LDS per phase */
+  int32_t max_patch_nnz;  /* max CSR entries of one patch (LDS slice size) */
+  int32_t max_row_len;    /* max CSR row length */
+  const int32_t* prow;       /* [n_patch+1] first row of each patch */
+  const int32_t* pelem_ptr;  /* [n_patch+1] offsets into pelem */
+  const int32_t* pelem;      /* patch element lists (element ids, owned first) */
+  const int32_t* inc_ptr;    /* [n_inc_rows+1] incidence list offsets, one list per assembly row */
+  const uint32_t* inc;       /* incidence items (packed slot/local index/column positions) */
+  const int32_t* inc_aux;    /* family-specific auxiliary table (NULL for P1) */
+  /* problem parameters */
+  double param[8];        /* p | eps | C1,D1 | E,nu,phi_deg,cohesion,reg */
+  const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]) */
+  int32_t nq;             /* quadrature points */
+  int32_t flags;          /* FNE_FLAG_* */
+} fne_plan_desc;
+
+#define FNE_FLAG_ABS_DET 1 /* Neo-Hookean: |det F| (serial path hyperelasticity/jax/jax_energy.py:26) */
+
+typedef struct fne_plan fne_plan;
+
+/* Replaces ReorderedElementAssemblerBase.__init__'s kernel build + scatter build
+ * (core/petsc/reordered_element_base.py:803-819). */
+int fne_plan_create(const fne_plan_desc* desc, fne_plan** out);
+int fne_plan_destroy(fne_plan* plan);
+/* Replaces update of model parameters between continuation steps (e.g. Davis-B reduced
+ * cohesion/friction, slope_stability/jax_petsc/solver.py:600-603). */
+int fne_plan_set_params(fne_plan* plan, const double* param8);
+/* Mohr-Coulomb only: 0 = full MC energy, 1 = elastic tangent/energy
+ * (slope_stability/jax_petsc/reordered_element_assembler.py:305-311). */
+int fne_plan_set_mode(fne_plan* plan, int32_t constitutive_mode);
+
+/*
+ * Energy: out[0] = sum_e w_e E_e(v) - sum_i f[i] x[i]  (FP64, compensated, deterministic).
+ * Replaces energy_fn (core/petsc/reordered_element_base.py:1579-1611).
+ * x: [n_free] free values, dvals: [n_total] Dirichlet values, f: [n_free] load or NULL,
+ * out: device FP64 scalar.
+ */
+int fne_energy(fne_plan* plan, const double* x, const double* dvals, const double* f,
+               double* out, fne_stream_t stream);
+
+/*
+ * Gradient: g[i] = sum over incident elements of dE_e/dv - f[i].
+ * Replaces gradient_fn (core/petsc/reordered_element_base.py:1613-1633).
+ */
+int fne_gradient(fne_plan* plan, const double* x, const double* dvals, const double* f,
+                 double* g, fne_stream_t stream);
+
+/*
+ * Hessian values into the plan's fixed CSR pattern: vals[k] for k < nnz, FP64 or FP32.
+ * Replaces assemble_hessian_element's JAX element Hessian + np.add.at scatter + setValuesCOO
+ * (core/petsc/reordered_element_base.py:1733-1775).  g may be non-NULL to receive the
+ * gradient from the same pass (fused gradient + Hessian).
+ */
+int fne_hessian(fne_plan* plan, const double* x, const double* dvals, const double* f,
+                double* g, void* vals, int32_t precision, fne_stream_t stream);
+
+/*
+ * Per-element energy / gradient / Hessian (dense, for the autograd boundary and tests):
+ * e_out[n_elem] (may be NULL), ge_out[n_loc][n_elem] (may be NULL),
+ * he_out[n_loc*n_loc][n_elem] (may be NULL).  Replaces the _make_local_element_kernels
+ * triple (reordered_element_base.py:922-923).
+ */
+int fne_element_eval(fne_plan* plan, const double* x, const double* dvals, double* e_out,
+                     double* ge_out, double* he_out, fne_stream_t stream);
+
+/*
+ * CSR SpMV y = A x, A values FP64 or FP32, x/y FP64.  Replaces PETSc MatMult
+ * (hessian_matvec_fn = A.mult, core/petsc/scalar_problem_driver.py:294).
+ */
+int fne_spmv(int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
+             int32_t precision, const double* x, double* y, fne_stream_t stream);
+
+/* CSR diagonal (Jacobi preconditioner set-up): diag[i] = A[i,i] (FP64 out). */
+int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
+                 int32_t precision, double* diag, fne_stream_t stream);
+
+/* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
+int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
+
+const char* fne_last_error(void);
+/* Library/ABI version string. */
+const char* fne_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FNE_HIP_H */

@@ -1,0 +1,155 @@
+"""Host set-up: permutations, CSR pattern and the P1 patch schedule (CPU only).
+
+The patch schedule is executed here by a numpy emulation of the kernel's phase B
+(incidence decode + fixed-order accumulation) and compared against the oracle's
+reference-style np.add.at assembly and a brute-force dense double loop.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from fenics_nonlinear_energies_amd import layout, patches
+from fenics_nonlinear_energies_amd.problems import mesh2d
+from fenics_nonlinear_energies_amd.problems.plaplace import build_plaplace_problem
+from oracle import assembly
+from tests import fixtures
+
+
+# --- reference unit tests (tests/test_reordered_element_base.py:12-73) -------------------
+def test_block_xyz_scalar_golden():
+    coords = np.array([[1.0, 0.0], [0.0, 1.0], [0.0, 0.0], [1.0, 1.0]])
+    perm = layout.select_permutation(
+        "block_xyz", adjacency=sp.eye(4, format="csr"), coords_all=coords, freedofs=np.arange(4), block_size=1
+    )
+    assert perm.tolist() == [2, 1, 0, 3]
+
+
+def test_block_xyz_vector_golden():
+    coords = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    perm = layout.select_permutation(
+        "block_xyz", adjacency=sp.eye(6, format="csr"), coords_all=coords, freedofs=np.arange(6), block_size=3
+    )
+    assert perm.tolist() == [3, 4, 5, 0, 1, 2]
+
+
+def test_local_vec_from_full_golden():
+    v = layout.local_vec_from_full(
+        np.array([10.0, 20.0, 30.0]),
+        np.array([-1, 0, 1, -1, 2]),
+        np.arange(5),
+        np.array([5.0, -1.0, -1.0, 6.0, -1.0]),
+    )
+    assert np.allclose(v, [5.0, 10.0, 20.0, 6.0, 30.0])
+
+
+def test_morton_is_permutation_and_keeps_node_components_adjacent():
+    coords = np.random.default_rng(1).random((50, 3))
+    free = np.array([3 * n + c for n in range(50) for c in range(3) if not (n % 7 == 0 and c == 1)])
+    perm = layout.perm_block_morton(coords, free, 3)
+    assert np.array_equal(np.sort(perm), np.arange(free.size))
+    nodes = free[perm] // 3
+    # each node's free components are consecutive in the new order
+    changes = np.count_nonzero(np.diff(nodes) != 0) + 1
+    assert changes == np.unique(nodes).size
+
+
+# --- mesh generator closed forms --------------------------------------------------------
+def test_structured_mesh_counts_and_area():
+    nodes, elems = mesh2d.structured_triangle_mesh(7, 5, 0.0, 2.0, -1.0, 1.0)
+    assert nodes.shape == (8 * 6, 2) and elems.shape == (2 * 35, 3)
+    dvx, dvy, vol = mesh2d.triangle_operators(nodes, elems)
+    assert np.isclose(vol.sum(), 4.0)
+    # gradients of the P1 basis sum to zero and reproduce linear functions exactly
+    assert np.allclose(dvx.sum(1), 0) and np.allclose(dvy.sum(1), 0)
+    lin = 3.0 * nodes[:, 0] - 2.0 * nodes[:, 1]
+    assert np.allclose((lin[elems] * dvx).sum(1), 3.0) and np.allclose((lin[elems] * dvy).sum(1), -2.0)
+
+
+def test_reference_mesh_geometry_consistent():
+    """The HDF5-decoded reference meshes: stored dvx/dvy/vol equal those recomputed from nodes."""
+    for loader, lev in ((fixtures.plaplace, 5), (fixtures.gl, 5)):
+        d, adj = loader(lev)
+        dvx, dvy, vol = mesh2d.triangle_operators(d["nodes"], d["elems"])
+        assert np.allclose(dvx, d["dvx"], rtol=1e-12, atol=1e-9)
+        assert np.allclose(dvy, d["dvy"], rtol=1e-12, atol=1e-9)
+        assert np.allclose(vol, d["vol"], rtol=1e-12)
+        assert adj.shape == (d["freedofs"].size,) * 2
+    d, _ = fixtures.plaplace(5)
+    # the stored load is the P1 load of f = -10 (lumped vol/3 per vertex)
+    f = mesh2d.lumped_nodal_load(d["elems"], d["vol"], -10.0, d["nodes"].shape[0])
+    assert np.allclose(f, d["f"], rtol=1e-12, atol=1e-15)
+
+
+# --- patch schedule ----------------------------------------------------------------------
+def emulate_p1_patch_assembly(plan, ef_new, ge_new, he_new, indptr, n_free):
+    """numpy re-statement of p1_patch_kernel's phase B."""
+    items = plan.inc.astype(np.int64)
+    rows = np.repeat(np.arange(n_free), np.diff(plan.inc_ptr))
+    slot = items & 0x1FFF
+    a = (items >> 13) & 3
+    patch = rows // plan.rows_per_patch
+    e = plan.pelem[plan.pelem_ptr[patch] + slot]
+    g = np.zeros(n_free)
+    np.add.at(g, rows, ge_new[e, a])
+    vals = np.zeros(int(indptr[-1]))
+    for b in range(3):
+        pos = (items >> (15 + 5 * b)) & 31
+        ok = pos != 31
+        np.add.at(vals, indptr[rows[ok]] + pos[ok], he_new[e[ok], a[ok], b])
+        # every free column of the element has a slot, every constrained one has none
+        assert np.all(ok == (ef_new[e, b] >= 0))
+    return g, vals
+
+
+@pytest.mark.parametrize("mode", ["block_morton", "block_xyz", "none"])
+@pytest.mark.parametrize("rows_per_patch", [64, 256])
+def test_p1_plan_reproduces_oracle_assembly(mode, rows_per_patch):
+    d, adj = fixtures.plaplace(5)
+    n_free = d["freedofs"].size
+    perm = layout.select_permutation(mode, adjacency=adj, coords_all=d["nodes"], freedofs=d["freedofs"], block_size=1)
+    lay = layout.build_global_layout(d, adj, perm, block_size=1, dirichlet_key="u_0")
+    ef = lay.total_to_free_reord[d["elems"]]
+    plan = patches.build_p1_plan(ef, lay.indptr, lay.indices, n_free, rows_per_patch=rows_per_patch, chunk=128)
+    # the patch lists hold exactly the (patch, element) pairs with a row of the element in the patch
+    ef_new = ef[plan.elem_order]
+    e_i, a_i = np.nonzero(ef_new >= 0)
+    want = np.unique((ef_new[e_i, a_i] // rows_per_patch) * ef.shape[0] + e_i)
+    got_p = np.repeat(np.arange(plan.n_patch), np.diff(plan.pelem_ptr))
+    got = np.sort(got_p.astype(np.int64) * ef.shape[0] + plan.pelem)
+    assert np.array_equal(want, got)
+    rng = np.random.default_rng(7)
+    u = rng.random(n_free)
+    ref = assembly.assemble("plaplace", d, u)
+    v = assembly.local_state(d, u)
+    q = assembly.element_eval("plaplace", d, v)
+    order = plan.elem_order
+    g_emul, vals = emulate_p1_patch_assembly(plan, ef[order], q["g"][order], q["h"][order], lay.indptr, n_free)
+    g_emul -= d["f"][d["freedofs"]][perm]
+    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))
+    # map back to the original numbering
+    Hp = H[lay.iperm][:, lay.iperm]
+    assert abs(Hp - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
+    assert np.allclose(g_emul[lay.iperm], ref["g"], rtol=1e-12, atol=1e-14)
+    # owned elements first, and each patch lists distinct elements
+    for p in range(plan.n_patch):
+        pe = plan.pelem[plan.pelem_ptr[p] : plan.pelem_ptr[p + 1]]
+        assert np.unique(pe).size == pe.size
+
+
+def test_oracle_assembly_matches_bruteforce():
+    d, _ = fixtures.plaplace(1)
+    u = np.random.default_rng(0).random(d["freedofs"].size)
+    H = assembly.assemble("plaplace", d, u)["h"].toarray()
+    Hb = assembly.dense_hessian_bruteforce("plaplace", d, u)
+    assert np.allclose(H, Hb, rtol=1e-14, atol=1e-14)
+    assert np.allclose(H, H.T, rtol=1e-13, atol=1e-14)
+
+
+def test_patch_halo_is_small_for_morton_on_structured_grid():
+    params, adj, _ = build_plaplace_problem(64)
+    n_free = params["freedofs"].size
+    perm = layout.select_permutation("block_morton", adjacency=adj, coords_all=params["nodes"], freedofs=params["freedofs"])
+    lay = layout.build_global_layout(params, adj, perm, block_size=1, dirichlet_key="u_0")
+    plan = patches.build_p1_plan(lay.total_to_free_reord[params["elems"]], lay.indptr, lay.indices, n_free)
+    assert plan.halo_factor < 1.35

@@ -1,0 +1,125 @@
+"""Pin the oracle's P1 element derivatives (p-Laplace, Ginzburg-Landau) without the reference.
+
+Checks: central finite differences along seeded random directions, an independent torch
+FP64 autodiff twin written from the reference integrands, textbook special cases
+(p = 2 / potential-free GL -> P1 Laplacian stiffness), and the reference meshes' own
+geometry data.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import elements
+from tests import fixtures
+
+
+def _mesh_arrays(n=40, seed=0):
+    d, _ = fixtures.plaplace(4)
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(d["elems"].shape[0], size=n, replace=False)
+    return d["dvx"][idx], d["dvy"][idx], d["vol"][idx], rng
+
+
+def _fd_check(fun, v, rng, h=1e-6, rtol=2e-6):
+    out = fun(v)
+    for _ in range(3):
+        dv = rng.standard_normal(v.shape)
+        ep = fun(v + h * dv)["e"]
+        em = fun(v - h * dv)["e"]
+        fd = (ep - em) / (2 * h)
+        an = np.sum(out["g"] * dv, axis=1)
+        assert np.allclose(fd, an, rtol=rtol, atol=1e-9 * np.abs(an).max())
+        gp = fun(v + h * dv)["g"]
+        gm = fun(v - h * dv)["g"]
+        fdg = (gp - gm) / (2 * h)
+        ang = np.einsum("nab,nb->na", out["h"], dv)
+        assert np.allclose(fdg, ang, rtol=rtol, atol=1e-7 * np.abs(ang).max())
+
+
+@pytest.mark.parametrize("p", [3.0, 2.5, 4.0])
+def test_plaplace_fd(p):
+    dvx, dvy, vol, rng = _mesh_arrays()
+    v = rng.standard_normal((dvx.shape[0], 3))
+    _fd_check(lambda vv: elements.plaplace(vv, dvx, dvy, vol, p), v, rng)
+
+
+def test_plaplace_p2_is_laplacian_stiffness():
+    dvx, dvy, vol, rng = _mesh_arrays()
+    v = rng.standard_normal((dvx.shape[0], 3))
+    h = elements.plaplace(v, dvx, dvy, vol, 2.0)["h"]
+    K = vol[:, None, None] * (dvx[:, :, None] * dvx[:, None, :] + dvy[:, :, None] * dvy[:, None, :])
+    assert np.allclose(h, K, rtol=1e-14, atol=0)
+    # Laplacian rows sum to zero (constants in the kernel)
+    assert np.allclose(h.sum(axis=2), 0.0, atol=1e-10 * np.abs(h).max())
+
+
+def test_plaplace_zero_state_hessian_limit_is_zero():
+    dvx, dvy, vol, _ = _mesh_arrays()
+    out = elements.plaplace(np.zeros((dvx.shape[0], 3)), dvx, dvy, vol, 3.0)
+    assert np.all(out["h"] == 0.0) and np.all(out["g"] == 0.0) and np.all(out["e"] == 0.0)
+
+
+def _torch_plaplace(v, dvx, dvy, vol, p):
+    fx = (v * dvx).sum(-1)
+    fy = (v * dvy).sum(-1)
+    return (1.0 / p) * (fx**2 + fy**2) ** (p / 2.0) * vol
+
+
+def _torch_gl(v, dvx, dvy, vol, ip, w, eps):
+    fx = (v * dvx).sum(-1)
+    fy = (v * dvy).sum(-1)
+    uq = v @ ip
+    return (0.5 * eps * (fx**2 + fy**2) + 0.25 * (((uq**2) - 1.0) ** 2) @ w) * vol
+
+
+def test_plaplace_matches_torch_autodiff():
+    dvx, dvy, vol, rng = _mesh_arrays(12)
+    v = rng.standard_normal((12, 3))
+    out = elements.plaplace(v, dvx, dvy, vol, 3.0)
+    for n in range(12):
+        args = [torch.tensor(a[n], dtype=torch.float64) for a in (dvx, dvy)] + [torch.tensor(vol[n])]
+        f = lambda vv: _torch_plaplace(vv, *args, 3.0)
+        vt = torch.tensor(v[n], dtype=torch.float64)
+        g = torch.func.grad(f)(vt).numpy()
+        H = torch.func.hessian(f)(vt).numpy()
+        assert np.allclose(out["g"][n], g, rtol=1e-13, atol=1e-14 * np.abs(g).max())
+        assert np.allclose(out["h"][n], H, rtol=1e-12, atol=1e-13 * np.abs(H).max())
+
+
+def test_gl_fd_and_torch_autodiff():
+    d, _ = fixtures.gl(5)
+    rng = np.random.default_rng(3)
+    idx = rng.choice(d["elems"].shape[0], size=24, replace=False)
+    dvx, dvy, vol = d["dvx"][idx], d["dvy"][idx], d["vol"][idx]
+    ip, w, eps = d["ip"], d["w"], float(d["eps"])
+    v = rng.uniform(-1.2, 1.2, size=(24, 3))
+    fun = lambda vv: elements.ginzburg_landau(vv, dvx, dvy, vol, ip, w, eps)
+    _fd_check(fun, v, rng)
+    out = fun(v)
+    for n in range(6):
+        targs = [torch.tensor(a[n], dtype=torch.float64) for a in (dvx, dvy)] + [torch.tensor(vol[n])]
+        f = lambda vv: _torch_gl(vv, *targs, torch.tensor(ip), torch.tensor(w), eps)
+        vt = torch.tensor(v[n], dtype=torch.float64)
+        assert np.allclose(out["e"][n], float(f(vt)), rtol=1e-14)
+        assert np.allclose(out["h"][n], torch.func.hessian(f)(vt).numpy(), rtol=1e-12, atol=1e-15)
+
+
+def test_gl_without_potential_is_eps_laplacian():
+    d, _ = fixtures.gl(2)
+    dvx, dvy, vol = d["dvx"], d["dvy"], d["vol"]
+    v = np.random.default_rng(0).standard_normal((dvx.shape[0], 3))
+    out = elements.ginzburg_landau(v, dvx, dvy, vol, d["ip"], np.zeros(3), 0.01)
+    K = vol[:, None, None] * (dvx[:, :, None] * dvx[:, None, :] + dvy[:, :, None] * dvy[:, None, :])
+    assert np.allclose(out["h"], 0.01 * K, rtol=1e-14, atol=0)
+
+
+def test_gl_quadrature_table_is_exact_p1_rule():
+    """The stored 3-point rule integrates quadratics exactly: ip = phi_a(xi_q), sum w = 1."""
+    d, _ = fixtures.gl(2)
+    ip, w = d["ip"], d["w"]
+    assert np.allclose(w.sum(), 1.0)
+    assert np.allclose(ip.sum(axis=0), 1.0)  # partition of unity at each point
+    # int phi_a phi_b over the reference triangle (area-normalised): (1 + delta_ab) / 12
+    M = np.einsum("aq,bq,q->ab", ip, ip, w)
+    assert np.allclose(M, (np.ones((3, 3)) + np.eye(3)) / 12.0)
