@@ -1,0 +1,188 @@
+"""Newton-step assembly benchmark (energy + gradient + CSR Hessian) on one MI355X.
+
+Workload: Ginzburg-Landau, eps = 0.01 on [-1,1]^2, structured 1024x1024 grid of P1
+triangles (2,097,152 elements, 1,046,529 free DOFs, 7,317,521 CSR nonzeros), initial
+guess 0.5 sin(pi x) sin(pi y) + U[-1e-3, 1e-3] (seed 0).  One step = one Newton-step
+assembly: the energy kernel plus the fused gradient + Hessian kernel (Hessian values FP32,
+element math and gradient FP64).  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (see the field comments below).  ``--gpus N`` with N > 1 exits: this
+build targets one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1024, help="cells per side")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=256)
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(n_cells: int, budget_s: float) -> dict:
+    """Oracle (numpy FP64, one thread) energy + gradient + Hessian assembly on a bounded sample."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import numpy as np
+
+    from fenics_nonlinear_energies_amd.problems.ginzburg_landau import build_gl_problem, sine_noise_init
+    from oracle import assembly
+
+    params, _ = build_gl_problem(n_cells)
+    u = sine_noise_init(params, 0)
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while True:
+        t0 = time.perf_counter()
+        assembly.assemble("gl", params, u)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 20:
+            break
+    med = float(np.median(times))
+    ne = params["elems"].shape[0]
+    return {
+        "value": ne / med,
+        "unit": "elem/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"GL {n_cells}x{n_cells} P1 ({ne} elements), numpy FP64 element energy+gradient+Hessian "
+        f"with np.add.at scatter into CSR, median of {len(times)} repeats",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from fenics_nonlinear_energies_amd.problems.ginzburg_landau import (
+        GLReorderedElementAssembler,
+        build_gl_problem,
+        sine_noise_init,
+    )
+
+    torch.cuda.set_device(0)
+    params, adj = build_gl_problem(args.n)
+    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision=args.precision)
+    u = sine_noise_init(params, 0)
+    x = asm.create_vec(u[asm.layout.perm])
+    g = x.duplicate()
+    y = x.duplicate()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        asm.energy_device(x.t)
+        asm.hessian_device(x.t, g.t)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region: K steps; per-launch HIP events around the dominant (fused grad+Hessian) kernel
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        asm.energy_device(x.t)
+        ev[k][0].record(stream)
+        asm.hessian_device(x.t, g.t)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    # SpMV (the Krylov hot loop) on the assembled Hessian
+    for _ in range(10):
+        asm.A.mult(x, y)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    n_sp = 100
+    for _ in range(n_sp):
+        asm.A.mult(x, y)
+    e.record(stream)
+    torch.cuda.synchronize()
+    spmv_ms = s.elapsed_time(e) / n_sp
+
+    ne = asm.n_elem
+    nnz = asm.layout.nnz
+    nf = asm.layout.n_free
+    nt = params["nodes"].shape[0]
+    s_h = 4 if args.precision == "fp32" else 8
+    # algorithmic bytes of one assembly: connectivity + FP64 geometry (7/elem) + nodal state +
+    # gradient write + CSR values write
+    b_asm = ne * (4 * 3 + 8 * 7) + 8 * nt + 8 * nf + s_h * nnz
+    b_spmv = nnz * (s_h + 4) + 4 * (nf + 1) + 8 * nf + 8 * nf
+    achieved = b_asm / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_gl{args.n}_{args.precision}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    ms_per_step = wall / args.steps * 1e3
+    out = {
+        "metric": "elem/s for energy+grad+Hessian CSR assembly",
+        "value": ne / (wall / args.steps),
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 element math, f32 Hessian values",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Ginzburg-Landau eps=0.01, {args.n}x{args.n} P1 grid, one Newton-step assembly "
+            f"(energy + fused gradient/Hessian), Hessian {args.precision}",
+            "n_elem": ne,
+            "n_free": nf,
+            "nnz": int(nnz),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "p1_patch_kernel (fused gradient + Hessian)",
+            "algorithmic_bytes_per_launch": int(b_asm),
+            "avg_launch_ms": kern_ms,
+        },
+        "spmv": {
+            "avg_ms": spmv_ms,
+            "achieved_GBs": b_spmv / (spmv_ms * 1e-3) / 1e9,
+            "frac": b_spmv / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes": int(b_spmv),
+        },
+    }
+    asm.cleanup()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_n, args.cpu_budget_s)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -93,14 +93,16 @@ class ReorderedElementAssemblerBase:
             raise ValueError(f"Unsupported local_hessian_mode={local_hessian_mode!r} (element only)")
         if assembly_backend not in {"coo", "coo_local", "blocked_local"}:
             raise ValueError(f"Unsupported assembly backend {assembly_backend!r}")
-        if not torch.cuda.is_available():
-            raise RuntimeError("ReorderedElementAssemblerBase needs a GPU (torch.cuda is unavailable)")
-        native.lib()  # fail loudly before any set-up work
+        host_only = str(device) == "host-plan"
+        if not host_only:
+            if not torch.cuda.is_available():
+                raise RuntimeError("ReorderedElementAssemblerBase needs a GPU (torch.cuda is unavailable)")
+            native.lib()  # fail loudly before any set-up work
         self.comm = comm if comm is not None else _SelfComm()
         self.rank = 0
         self.size = 1
         self.params = params
-        self.device = torch.device(device)
+        self.device = None if host_only else torch.device(device)
         self.reorder_mode = str(reorder_mode)
         self.local_hessian_mode = str(local_hessian_mode)
         self.use_near_nullspace = bool(use_near_nullspace)
@@ -148,8 +150,14 @@ class ReorderedElementAssemblerBase:
         self.dirichlet_full = np.asarray(params[self.dirichlet_key], dtype=np.float64).ravel().copy()
 
         t0 = time.perf_counter()
-        self._build_device_plan(rows_per_patch=int(rows_per_patch), chunk=chunk)
+        self._build_host_plan(rows_per_patch=rows_per_patch, chunk=chunk)
         self._setup_timings["scatter_build"] = time.perf_counter() - t0
+        if host_only:
+            # CPU-side planning only (tests, plan inspection): no device state is created
+            return
+        t0 = time.perf_counter()
+        self._upload_plan()
+        self._setup_timings["device_upload"] = time.perf_counter() - t0
 
         t0 = time.perf_counter()
         self._f_owned = np.asarray(self._build_rhs_owned(), dtype=np.float64)
@@ -188,13 +196,22 @@ class ReorderedElementAssemblerBase:
         """LDS staged per element in the patch kernel (P1: 6 Hessian + 3 gradient FP64)."""
         return 72
 
-    def _build_patch_plan(self, elem_free, rows_per_patch: int, chunk) -> PatchPlan:
+    def _lds_slice_bytes_per_entry(self) -> int:
+        """LDS bytes per CSR entry of the patch slice (P1 accumulates in FP64)."""
+        return 8
+
+    @staticmethod
+    def lds_budget_bytes() -> int:
+        """Dynamic LDS one patch workgroup uses (64 KiB: several workgroups per CU)."""
+        return 64 * 1024
+
+    def _build_patch_plan(self, elem_free, rows_per_patch, chunk) -> PatchPlan:
         return build_p1_plan(
             elem_free,
             self.layout.indptr,
             self.layout.indices,
             self.layout.n_free,
-            rows_per_patch=rows_per_patch,
+            rows_per_patch=int(rows_per_patch or 256),
             chunk=int(chunk or 512),
         )
 
@@ -204,15 +221,15 @@ class ReorderedElementAssemblerBase:
     def _t(self, a, dtype) -> torch.Tensor:
         return torch.as_tensor(np.ascontiguousarray(a, dtype=dtype)).to(self.device)
 
-    def _build_device_plan(self, rows_per_patch: int, chunk) -> None:
+    def _build_host_plan(self, rows_per_patch, chunk) -> None:
         lay = self.layout
         elems = np.asarray(self.params["elems"], dtype=np.int64)
         elem_free = lay.total_to_free_reord[elems]
         plan = self._build_patch_plan(elem_free, rows_per_patch, chunk)
-        # LDS budget of one workgroup: CSR slice (FP64) + staged element data per chunk
-        budget = 64 * 1024
+        # LDS budget of one workgroup: CSR slice + staged element data per chunk
+        budget = self.lds_budget_bytes()
         per_elem = self._lds_bytes_per_element()
-        avail = budget - 8 * int(plan.max_patch_nnz)
+        avail = budget - self._lds_slice_bytes_per_entry() * int(plan.max_patch_nnz) - 64
         if avail < 64 * per_elem:
             raise ValueError("patch CSR slice too large for LDS; reduce rows_per_patch")
         if plan.chunk * per_elem > avail:
@@ -220,10 +237,16 @@ class ReorderedElementAssemblerBase:
         self.patch_plan = plan
         order = plan.elem_order
         n_free = lay.n_free
-        codes = np.where(elem_free >= 0, elem_free, n_free + elems)[order]
-        geo = self._element_geometry()[:, order]
+        self._host_codes = np.where(elem_free >= 0, elem_free, n_free + elems)[order]
+        self._host_geo = self._element_geometry()[:, order]
         self.n_elem = int(elems.shape[0])
         self.n_loc = int(elems.shape[1])
+
+    def _upload_plan(self) -> None:
+        lay = self.layout
+        plan = self.patch_plan
+        n_free = lay.n_free
+        codes, geo = self._host_codes, self._host_geo
         self._dev = {
             "edof": self._t(codes.T, np.int32),
             "egeo": self._t(geo, np.float64),

@@ -148,3 +148,102 @@ def build_p1_plan(
         max_patch_nnz=int(patch_nnz.max()) if patch_nnz.size else 0,
         max_row_len=max_row_len,
     )
+
+
+BLOCK_SLOT_BITS = 16
+BLOCK_POS_NONE = 255
+
+
+def build_block_plan(
+    elem_free: np.ndarray,
+    indptr: np.ndarray,
+    indices: np.ndarray,
+    n_free: int,
+    *,
+    block_size: int,
+    nodes_per_patch: int = 64,
+    chunk: int = 128,
+) -> PatchPlan:
+    """Patch plan for vector elements whose nodes are fully free or fully constrained.
+
+    Rows come in node blocks of ``block_size`` consecutive free DOFs.  Incidence lists are
+    per node row; an item (uint64) is slot[0:16) | a[16:18) | bpos_b[24+8b : 32+8b), where
+    bpos_b is the block column of the element's node b inside the node's block row (255 for
+    a constrained node).  Supports up to 4 nodes per element (P1 tetrahedra).
+    """
+    bs = int(block_size)
+    elem_free = np.asarray(elem_free, dtype=np.int64)
+    n_elem, n_loc = elem_free.shape
+    n_nodes_e = n_loc // bs
+    if n_nodes_e > 4 or n_nodes_e * bs != n_loc:
+        raise ValueError("build_block_plan supports elements with <= 4 nodes")
+    if n_free % bs:
+        raise ValueError("free DOFs must come in whole node blocks")
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    node = elem_free[:, 0::bs]
+    free_node = node >= 0
+    # node rows: all components free and consecutive, or all constrained
+    for i in range(bs):
+        comp = elem_free[:, i::bs]
+        if not np.array_equal(comp >= 0, free_node):
+            raise ValueError("mixed free/constrained node components are not supported by the block plan")
+        if np.any(comp[free_node] != node[free_node] + i):
+            raise ValueError("node components are not consecutive in the free-DOF order")
+        if np.any(node[free_node] % bs):
+            raise ValueError("node blocks are not aligned to block_size")
+    nrow = np.where(free_node, node // bs, -1)
+    n_nrows = n_free // bs
+    # block pattern from the scalar CSR (first row of each node block)
+    brow_len = (indptr[bs * np.arange(n_nrows) + 1] - indptr[bs * np.arange(n_nrows)]) // bs
+    bptr = np.zeros(n_nrows + 1, dtype=np.int64)
+    bptr[1:] = np.cumsum(brow_len)
+    first = indptr[bs * np.arange(n_nrows)]
+    sel = np.repeat(first, brow_len) + bs * (np.arange(bptr[-1]) - np.repeat(bptr[:-1], brow_len))
+    bcol = indices[sel] // bs
+    if int(brow_len.max(initial=0)) >= BLOCK_POS_NONE:
+        raise ValueError("node degree too large for 8-bit block positions")
+
+    prow_nodes = _patch_rows(n_nrows, nodes_per_patch)
+    n_patch = prow_nodes.size - 1
+    elem_order, _, nr, pelem, pelem_ptr, pair_key, slot_sorted = _elements_by_patch(nrow, nodes_per_patch, n_patch)
+    if np.diff(pelem_ptr).max(initial=0) >= (1 << BLOCK_SLOT_BITS):
+        raise ValueError("too many elements in one patch")
+    e_i, a_i = np.nonzero(nr >= 0)
+    na = nr[e_i, a_i]
+    p_i = na // nodes_per_patch
+    slot = slot_sorted[np.searchsorted(pair_key, p_i * np.int64(n_elem) + e_i)]
+    bkeys = np.repeat(np.arange(n_nrows, dtype=np.int64), brow_len) * np.int64(n_nrows) + bcol
+    item = slot.astype(np.uint64) | (a_i.astype(np.uint64) << np.uint64(16))
+    for b in range(n_nodes_e):
+        nb = nr[e_i, b]
+        pos = np.full(e_i.size, BLOCK_POS_NONE, dtype=np.int64)
+        ok = nb >= 0
+        key = na[ok] * np.int64(n_nrows) + nb[ok]
+        k = np.searchsorted(bkeys, key)
+        if not np.array_equal(bkeys[k], key):
+            raise RuntimeError("element node coupling missing from the block pattern")
+        pos[ok] = k - bptr[na[ok]]
+        item |= pos.astype(np.uint64) << np.uint64(24 + 8 * b)
+    for b in range(n_nodes_e, 4):
+        item |= np.uint64(BLOCK_POS_NONE) << np.uint64(24 + 8 * b)
+    order = np.lexsort((slot, na))
+    inc = item[order]
+    inc_ptr = np.zeros(n_nrows + 1, dtype=np.int64)
+    inc_ptr[1:] = np.cumsum(np.bincount(na, minlength=n_nrows))
+    prow = bs * prow_nodes
+    patch_nnz = indptr[prow[1:]] - indptr[prow[:-1]]
+    return PatchPlan(
+        n_patch=n_patch,
+        rows_per_patch=bs * nodes_per_patch,
+        chunk=int(chunk),
+        prow=prow.astype(np.int32),
+        pelem_ptr=pelem_ptr.astype(np.int32),
+        pelem=pelem.astype(np.int32),
+        elem_order=elem_order,
+        inc_ptr=inc_ptr.astype(np.int32),
+        inc=inc,
+        inc_aux=None,
+        max_patch_nnz=int(patch_nnz.max()) if patch_nnz.size else 0,
+        max_row_len=int(np.diff(indptr).max(initial=0)),
+    )

@@ -66,6 +66,15 @@ class PLaplaceReorderedElementAssembler(ReorderedElementAssemblerBase):
     def _kernel_params(self) -> list[float]:
         return [float(self.params["p"])]
 
+    def assemble_first_step_hessian(self, u_owned):
+        """Assemble the p = 2 (Laplacian) Hessian into ``A``: first-step policy for u = 0 starts."""
+        p = float(self.params["p"])
+        self.set_kernel_params([2.0])
+        try:
+            return self.assemble_hessian(u_owned)
+        finally:
+            self.set_kernel_params([p])
+
     def _build_rhs_owned(self) -> np.ndarray:
         rhs = np.asarray(self.params["f"], dtype=np.float64)
         freedofs = np.asarray(self.params["freedofs"], dtype=np.int64)

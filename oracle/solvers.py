@@ -1,0 +1,240 @@
+"""Oracle Krylov solvers and Newton loop (numpy, FP64) -- TEST INFRASTRUCTURE ONLY.
+
+* ``cg``: PETSc KSPCG semantics (zero initial guess, left Jacobi preconditioning with zero
+  diagonal entries replaced by 1, convergence on the preconditioned residual norm
+  ||B r_k|| <= rtol ||B r_0||, indefinite-curvature stop, iteration cap).
+* ``gmres``: PETSc KSPGMRES(restart) with left preconditioning, modified Gram-Schmidt.
+* ``newton_golden``: the line-search branch of core/petsc/minimizers.py:181-1038
+  (golden-section search on [a, b] after non-finite interval repair, convergence on
+  |dE| < tolf and small step, then a gradient check when require_all_convergence).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def jacobi_inverse(A) -> np.ndarray:
+    d = A.diagonal().astype(np.float64)
+    d[d == 0.0] = 1.0
+    return 1.0 / d
+
+
+def cg(A, b, rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, inv_diag=None):
+    """Returns (x, its, reason)."""
+    inv_diag = jacobi_inverse(A) if inv_diag is None else inv_diag
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = inv_diag * r
+    dp = math.sqrt(float(z @ z))
+    r0 = dp
+    if dp <= max(rtol * r0, atol):
+        return x, 0, 2
+    beta = float(z @ r)
+    p = z.copy()
+    dpiold = None
+    betaold = beta
+    for i in range(max_it):
+        if i > 0:
+            p = z + (beta / betaold) * p
+        w = A @ p
+        dpi = float(p @ w)
+        if dpi == 0.0 or (dpiold is not None and dpi * dpiold <= 0.0) or not math.isfinite(dpi):
+            return x, i, -10
+        dpiold = dpi
+        a = beta / dpi
+        x += a * p
+        r -= a * w
+        z = inv_diag * r
+        dp = math.sqrt(float(z @ z))
+        if dp <= max(rtol * r0, atol):
+            return x, i + 1, 2
+        if dp >= dtol * r0:
+            return x, i + 1, -4
+        betaold = beta
+        beta = float(z @ r)
+    return x, max_it, -3
+
+
+def gmres(A, b, rtol=1e-3, max_it=10000, restart=30, atol=1e-50, inv_diag=None):
+    inv_diag = jacobi_inverse(A) if inv_diag is None else inv_diag
+    n = b.size
+    x = np.zeros(n)
+    its = 0
+    r0 = None
+    while True:
+        r = inv_diag * (b - A @ x) if its else inv_diag * b
+        beta = math.sqrt(float(r @ r))
+        if r0 is None:
+            r0 = beta
+            if beta <= max(rtol * r0, atol):
+                return x, 0, 2
+        if beta == 0.0:
+            return x, its, 3
+        m = restart
+        V = np.zeros((m + 1, n))
+        V[0] = r / beta
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        g[0] = beta
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        k_used = 0
+        done = 0
+        for k in range(m):
+            w = inv_diag * (A @ V[k])
+            for j in range(k + 1):
+                H[j, k] = float(w @ V[j])
+                w -= H[j, k] * V[j]
+            hn = math.sqrt(float(w @ w))
+            H[k + 1, k] = hn
+            if hn > 0.0:
+                V[k + 1] = w / hn
+            for j in range(k):
+                t0 = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = t0
+            den = math.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = (1.0, 0.0) if den == 0.0 else (H[k, k] / den, H[k + 1, k] / den)
+            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            k_used = k + 1
+            its += 1
+            res = abs(g[k + 1])
+            if res <= max(rtol * r0, atol):
+                done = 2
+            if done or its >= max_it or hn == 0.0:
+                break
+        y = np.zeros(k_used)
+        for i in range(k_used - 1, -1, -1):
+            s = g[i] - float(H[i, i + 1 : k_used] @ y[i + 1 : k_used])
+            y[i] = s / H[i, i] if H[i, i] != 0.0 else 0.0
+        x += y @ V[:k_used]
+        if done:
+            return x, its, done
+        if its >= max_it:
+            return x, its, -3
+
+
+def golden_section(f, a, b, tol):
+    """core/petsc/minimizers.py:14-62."""
+    gam = 0.5 + math.sqrt(5.0) / 2.0
+    an, bn = float(a), float(b)
+    fa, fb = f(an), f(bn)
+    dn = (bn - an) / gam + an
+    cn = an + bn - dn
+    fc, fd = f(cn), f(dn)
+    n = 4
+    best, bestv = an, fa
+    for al, v in ((bn, fb), (cn, fc), (dn, fd)):
+        if v < bestv:
+            best, bestv = al, v
+    while bn - an > tol:
+        if fc < fd:
+            bn = dn
+            dn, cn = cn, an + bn - cn
+            fd, fc = fc, f(cn)
+            cand = ((dn, fd), (cn, fc))
+        else:
+            an = cn
+            cn, dn = dn, an + bn - dn
+            fc, fd = fd, f(dn)
+            cand = ((cn, fc), (dn, fd))
+        for al, v in cand:
+            if v < bestv:
+                best, bestv = al, v
+        n += 1
+    mid = 0.5 * (an + bn)
+    n += 1
+    if f(mid) < bestv:
+        best = mid
+    return best, n
+
+
+def newton_golden(energy, gradient, solve, x0, *, tolf=1e-5, tolg=1e-3, tolx_rel=1e-6, tolx_abs=1e-10,
+                  maxit=100, ls=(-0.5, 2.0), ls_tol=1e-3, require_all=True, first_solve=None):
+    """Line-search Newton (golden section).  solve(x, rhs) -> (direction, its).
+
+    Non-finite energies along the line count as +inf; interval repair bisects each non-finite
+    end toward alpha = 0 (core/petsc/minimizers.py:74-126).
+    """
+    x = np.array(x0, dtype=np.float64)
+    fx = energy(x)
+    nit = 0
+    msg = "Maximum number of iterations reached"
+    lin = []
+    for _ in range(maxit):
+        if not np.isfinite(fx):
+            msg = f"Non-finite energy before Newton iteration {nit + 1}"
+            break
+        g = gradient(x)
+        ng = float(np.linalg.norm(g))
+        if not require_all and ng < tolg:
+            msg = "Gradient norm converged"
+            break
+        nit += 1
+        s = first_solve if (first_solve is not None and nit == 1) else solve
+        h, its = s(x, -g)
+        lin.append(its)
+        x_prev = x.copy()
+        xn = float(np.linalg.norm(x_prev))
+        fold = fx
+
+        def f_at(al):
+            v = energy(x_prev + al * h)
+            return v if np.isfinite(v) else np.inf
+
+        pn = float(np.linalg.norm(h))
+        accepted, dE, step = False, 0.0, 0.0
+        if pn > 1e-20:
+            a, b = ls
+
+            def repair(bound, sign):
+                if (sign < 0 and not bound < 0.0) or (sign > 0 and not bound > 0.0):
+                    return bound
+                if np.isfinite(f_at(bound)):
+                    return bound
+                good, bad = 0.0, bound
+                for _ in range(60):
+                    mid = 0.5 * (good + bad)
+                    if np.isfinite(f_at(mid)):
+                        good = mid
+                    else:
+                        bad = mid
+                    if abs(bad - good) <= max(ls_tol, 1e-12):
+                        break
+                return good
+
+            if np.isfinite(fold):
+                a, b = repair(a, -1.0), repair(b, 1.0)
+            al, _ = golden_section(f_at, a, b, ls_tol)
+            ft = f_at(al)
+            if np.isfinite(ft) and ft < fold:
+                x = x_prev + al * h
+                fx = ft
+                dE = fold - ft
+                step = abs(al) * pn
+                accepted = True
+            else:
+                x = x_prev
+        conv_e = accepted and abs(dE) < tolf
+        conv_s = accepted and (step < tolx_abs or step / max(1.0, xn) < tolx_rel)
+        post = np.nan
+        if require_all and conv_e and conv_s:
+            post = float(np.linalg.norm(gradient(x)))
+        if not np.isfinite(fx):
+            msg = f"Non-finite energy after update at Newton iteration {nit}"
+            x = x_prev
+            break
+        if require_all:
+            if conv_e and conv_s and post < tolg:
+                msg = "Converged (energy, step, gradient)"
+                break
+        elif conv_e:
+            msg = "Energy change converged"
+            break
+    return {"x": x, "fun": fx, "nit": nit, "message": msg, "linear_iters": lin}

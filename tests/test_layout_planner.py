@@ -153,3 +153,67 @@ def test_patch_halo_is_small_for_morton_on_structured_grid():
     lay = layout.build_global_layout(params, adj, perm, block_size=1, dirichlet_key="u_0")
     plan = patches.build_p1_plan(lay.total_to_free_reord[params["elems"]], lay.indptr, lay.indices, n_free)
     assert plan.halo_factor < 1.35
+
+
+def emulate_block_patch_assembly(plan, ef_new, ge_new, he_new, indptr, n_free, bs):
+    """numpy re-statement of tet_patch_kernel's phase B (scalar row = node row * bs + comp)."""
+    items = plan.inc.astype(np.uint64)
+    nrows = np.repeat(np.arange(n_free // bs), np.diff(plan.inc_ptr))
+    slot = (items & np.uint64(0xFFFF)).astype(np.int64)
+    a = ((items >> np.uint64(16)) & np.uint64(3)).astype(np.int64)
+    patch = (nrows * bs) // plan.rows_per_patch
+    e = plan.pelem[plan.pelem_ptr[patch] + slot]
+    n_nodes_e = ef_new.shape[1] // bs
+    g = np.zeros(n_free)
+    vals = np.zeros(int(indptr[-1]))
+    for comp in range(bs):
+        r = nrows * bs + comp
+        np.add.at(g, r, ge_new[e, bs * a + comp])
+        for b in range(n_nodes_e):
+            bp = ((items >> np.uint64(24 + 8 * b)) & np.uint64(255)).astype(np.int64)
+            ok = bp != 255
+            for k in range(bs):
+                np.add.at(vals, indptr[r[ok]] + bs * bp[ok] + k, he_new[e[ok], bs * a[ok] + comp, bs * b + k])
+    return g, vals
+
+
+def test_block_plan_reproduces_oracle_assembly_he():
+    from fenics_nonlinear_energies_amd.problems.hyperelasticity import build_beam_problem
+
+    p, adj, u0 = build_beam_problem(12, 3, 2)
+    n_free = p["freedofs"].size
+    perm = layout.select_permutation("block_morton", adjacency=adj, coords_all=p["nodes2coord"], freedofs=p["freedofs"], block_size=3)
+    lay = layout.build_global_layout(p, adj, perm, block_size=3, dirichlet_key="u_0_ref")
+    ef = lay.total_to_free_reord[p["elems"]]
+    plan = patches.build_block_plan(ef, lay.indptr, lay.indices, n_free, block_size=3, nodes_per_patch=10, chunk=32)
+    u = u0 + 1e-4 * np.random.default_rng(3).standard_normal(n_free)
+    ref = assembly.assemble("he", p, u, dirichlet_key="u_0_ref")
+    v = assembly.local_state(p, u, "u_0_ref")
+    q = assembly.element_eval("he", p, v)
+    order = plan.elem_order
+    g, vals = emulate_block_patch_assembly(plan, ef[order], q["g"][order], q["h"][order], lay.indptr, n_free, 3)
+    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))[lay.iperm][:, lay.iperm]
+    assert abs(H - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
+    assert np.allclose(g[lay.iperm], ref["g"], rtol=1e-10, atol=1e-10 * np.abs(ref["g"]).max())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_host_plans_fit_kernel_limits(prec):
+    """Assembler set-up without a GPU (device='host-plan'): LDS request within budget, limits met."""
+    from fenics_nonlinear_energies_amd.problems.ginzburg_landau import GLReorderedElementAssembler, build_gl_problem
+    from fenics_nonlinear_energies_amd.problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
+
+    gp, gadj = build_gl_problem(40)
+    hp, hadj, _ = build_beam_problem(24, 3, 3)
+    for cls, p, adj in ((GLReorderedElementAssembler, gp, gadj), (HEReorderedElementAssembler, hp, hadj)):
+        asm = cls(p, None, adj, hessian_precision=prec, device="host-plan")
+        plan = asm.patch_plan
+        lds = asm._lds_slice_bytes_per_entry() * plan.max_patch_nnz + asm._lds_bytes_per_element() * plan.chunk
+        assert lds <= asm.lds_budget_bytes()
+        assert plan.rows_per_patch <= 256 and plan.chunk >= 64
+        assert np.all(np.diff(plan.prow) <= 256)
+        assert asm._host_codes.shape == (p["elems"].shape[0], p["elems"].shape[1])
+        # every element code is a free index or n_free + a Dirichlet slot
+        n_free = asm.layout.n_free
+        n_total = asm.dirichlet_full.size
+        assert asm._host_codes.min() >= 0 and asm._host_codes.max() < n_free + n_total
