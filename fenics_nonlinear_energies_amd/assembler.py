@@ -28,7 +28,7 @@ import torch
 from . import native
 from .layout import build_global_layout, select_permutation, validate_permutation
 from .linalg import KSP, DeviceCSR
-from .patches import PatchPlan, build_p1_plan
+from .patches import PatchPlan, build_p1_colored_plan, build_p1_plan, build_p1_rowgather_plan
 from .vec import DeviceVec
 
 
@@ -83,8 +83,10 @@ class ReorderedElementAssemblerBase:
         petsc_log_events=False,
         memory_guard_total_gib=None,
         hessian_precision="fp64",
-        rows_per_patch=256,
+        assembly_mode="staged",
+        rows_per_patch=None,
         chunk=None,
+        lds_budget=None,
         device="cuda",
     ):
         if comm is not None and int(comm.Get_size()) != 1:
@@ -108,6 +110,8 @@ class ReorderedElementAssemblerBase:
         self.use_near_nullspace = bool(use_near_nullspace)
         self.assembly_backend = str(assembly_backend)
         self.hessian_precision = str(hessian_precision)
+        self.assembly_mode = str(assembly_mode)
+        self._lds_budget = None if lds_budget is None else int(lds_budget)
         self.memory_guard_total_gib = memory_guard_total_gib
         self.iter_timings: list[dict] = []
         self._hvp_eval_mode = "element_patch_lds"
@@ -187,7 +191,11 @@ class ReorderedElementAssemblerBase:
         return None
 
     def _plan_flags(self) -> int:
-        return 0
+        if self.block_size != 1:
+            return 0
+        if self.assembly_mode == "rowgather":
+            return native.FNE_FLAG_ROW_GATHER
+        return native.FNE_FLAG_STAGED if self.assembly_mode == "staged" else native.FNE_FLAG_COLORED
 
     def _build_rhs_owned(self) -> np.ndarray:
         return np.zeros(self.layout.n_free, dtype=np.float64)
@@ -196,24 +204,43 @@ class ReorderedElementAssemblerBase:
         """LDS staged per element in the patch kernel (P1: 6 Hessian + 3 gradient FP64)."""
         return 72
 
+    def _single_pass_staging(self) -> bool:
+        """True when the P1 kernel holds every patch element at once (colored: registers, staged: LDS)."""
+        return self.assembly_mode in ("patch", "staged") and self.block_size == 1
+
     def _lds_slice_bytes_per_entry(self) -> int:
         """LDS bytes per CSR entry of the patch slice (P1 accumulates in FP64)."""
         return 8
 
-    @staticmethod
-    def lds_budget_bytes() -> int:
-        """Dynamic LDS one patch workgroup uses (64 KiB: several workgroups per CU)."""
-        return 64 * 1024
+    def lds_budget_bytes(self) -> int:
+        """Dynamic LDS one patch workgroup may use (default 64 KiB: two or more workgroups per CU)."""
+        return self._lds_budget if getattr(self, "_lds_budget", None) else 64 * 1024
 
     def _build_patch_plan(self, elem_free, rows_per_patch, chunk) -> PatchPlan:
-        return build_p1_plan(
-            elem_free,
-            self.layout.indptr,
-            self.layout.indices,
-            self.layout.n_free,
-            rows_per_patch=int(rows_per_patch or 256),
-            chunk=int(chunk or 512),
-        )
+        if self.assembly_mode == "rowgather":
+            return build_p1_rowgather_plan(elem_free, self.layout.indptr, self.layout.indices, self.layout.n_free)
+        if self.assembly_mode == "staged":
+            plan = build_p1_plan(
+                elem_free, self.layout.indptr, self.layout.indices, self.layout.n_free,
+                rows_per_patch=int(rows_per_patch or 208), chunk=64,
+            )
+            plan.chunk = int(-(-max(plan.max_patch_elems, 1) // 64) * 64)  # stage every patch element
+            return plan
+        if self.assembly_mode != "patch":
+            raise ValueError(f"unknown assembly_mode {self.assembly_mode!r} (patch | staged | rowgather)")
+        # colored patch plan: largest row patch whose elements fit elems_per_thread * 256
+        ept = int(chunk or 2)
+        rpp = int(rows_per_patch or 256)
+        while True:
+            try:
+                return build_p1_colored_plan(
+                    elem_free, self.layout.indptr, self.layout.indices, self.layout.n_free,
+                    rows_per_patch=rpp, elems_per_thread=ept,
+                )
+            except ValueError as exc:
+                if "more elements" not in str(exc) or rpp <= 16:
+                    raise
+                rpp -= 16
 
     # ------------------------------------------------------------------------------------
     # device plan
@@ -228,11 +255,16 @@ class ReorderedElementAssemblerBase:
         plan = self._build_patch_plan(elem_free, rows_per_patch, chunk)
         # LDS budget of one workgroup: CSR slice + staged element data per chunk
         budget = self.lds_budget_bytes()
+        if plan.n_patch == 0 or (self._single_pass_staging() and self.assembly_mode == "patch"):
+            budget = 1 << 30  # no LDS element staging
         per_elem = self._lds_bytes_per_element()
-        avail = budget - self._lds_slice_bytes_per_entry() * int(plan.max_patch_nnz) - 64
-        if avail < 64 * per_elem:
-            raise ValueError("patch CSR slice too large for LDS; reduce rows_per_patch")
+        slice_bytes = 0 if self.assembly_mode == "staged" else self._lds_slice_bytes_per_entry() * int(plan.max_patch_nnz)
+        avail = budget - slice_bytes - 64
         if plan.chunk * per_elem > avail:
+            if self._single_pass_staging():
+                raise ValueError("patch elements do not fit the LDS budget; reduce rows_per_patch")
+            if avail < 64 * per_elem:
+                raise ValueError("patch CSR slice too large for LDS; reduce rows_per_patch")
             plan.chunk = int(max(64, (avail // per_elem) // 64 * 64))
         self.patch_plan = plan
         order = plan.elem_order
@@ -260,6 +292,14 @@ class ReorderedElementAssemblerBase:
         }
         if plan.inc_aux is not None:
             self._dev["inc_aux"] = self._t(plan.inc_aux, np.int32)
+        if plan.n_patch > 0:
+            pe = plan.pelem.astype(np.int64)
+            self._dev["pedof"] = self._t(codes[pe].T, np.int32)
+            self._dev["pgeo"] = self._t(geo[:, pe], np.float64)
+            self._dev["pown"] = self._t(plan.pown, np.int32)
+            if getattr(plan, "pitem", None) is not None:
+                self._dev["pitem"] = self._t(plan.pitem.view(np.int64), np.int64)
+                self._dev["pcolors"] = self._t(plan.pcolors, np.int32)
         q = self._quadrature_table()
         if q is not None:
             self._dev["qdata"] = self._t(q, np.float64)
@@ -288,6 +328,12 @@ class ReorderedElementAssemblerBase:
         d.inc_ptr = native.ptr(self._dev["inc_ptr"])
         d.inc = native.ptr(self._dev["inc"])
         d.inc_aux = native.ptr(self._dev.get("inc_aux"))
+        d.n_pe = int(plan.pelem.size) if plan.n_patch > 0 else 0
+        d.pedof = native.ptr(self._dev.get("pedof"))
+        d.pgeo = native.ptr(self._dev.get("pgeo"))
+        d.pown = native.ptr(self._dev.get("pown"))
+        d.pitem = native.ptr(self._dev.get("pitem"))
+        d.pcolors = native.ptr(self._dev.get("pcolors"))
         prm = list(self._kernel_params()) + [0.0] * 8
         for i in range(8):
             d.param[i] = float(prm[i])
@@ -389,6 +435,26 @@ class ReorderedElementAssemblerBase:
             ),
             "fne_hessian",
         )
+
+    def energy_grad_hessian_device(self, x: torch.Tensor, g: torch.Tensor | None = None,
+                                   out: torch.Tensor | None = None) -> torch.Tensor:
+        """One fused pass: energy (device scalar), gradient (optional) and Hessian into ``A``."""
+        out = self._energy_out if out is None else out
+        native.check(
+            native.lib().fne_energy_grad_hessian(
+                self._plan,
+                native.ptr(x),
+                native.ptr(self.dvals),
+                native.ptr(self._f_dev),
+                native.ptr(out),
+                native.ptr(g),
+                native.ptr(self.A.data),
+                self.A.fne_precision,
+                native.stream_ptr(),
+            ),
+            "fne_energy_grad_hessian",
+        )
+        return out
 
     def assemble_hessian(self, u_owned, variant=2):
         del variant

@@ -36,7 +36,7 @@ struct fne_plan {
   size_t max_lds = 65536;  // dynamic LDS bytes one workgroup may request on this device
 };
 
-constexpr int kEnergyBlocks = 1024;  // fixed grid => fixed reduction order => deterministic
+constexpr int kEnergyBlocks = 2048;  // fixed grid => fixed reduction order => deterministic
 constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------------------------
@@ -66,9 +66,10 @@ __device__ __forceinline__ void acc_merge(double& s, double& c, double s2, doubl
 
 // Deterministic block reduction of (s, c) pairs: fixed xor-shuffle tree inside each 64-wide
 // wavefront, then wave 0 merges the per-wave results in wave order.
+template <int NT = kBlock>
 __device__ __forceinline__ void block_reduce_pair(double& s, double& c) {
-  __shared__ double sh_s[kBlock / 64];
-  __shared__ double sh_c[kBlock / 64];
+  __shared__ double sh_s[NT / 64];
+  __shared__ double sh_c[NT / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double s2 = __shfl_xor(s, off, 64);
@@ -111,7 +112,7 @@ __device__ __forceinline__ double gather_dof(int c, int n_free, const double* __
 }
 
 // family launchers (one translation unit per family)
-enum FneOp { kOpEnergy = 0, kOpGradient = 1, kOpHessian = 2, kOpElementEval = 3 };
+enum FneOp { kOpEnergy = 0, kOpGradient = 1, kOpHessian = 2, kOpElementEval = 3, kOpFused = 4 };
 
 struct FneCall {
   const double* x;

@@ -8,11 +8,22 @@ namespace {
 thread_local std::string g_last_error;
 
 // Final deterministic reduction of the per-block (sum, compensation) partials.
-__global__ __launch_bounds__(kBlock) void finalize_pairs_kernel(const double2* __restrict__ partial,
-                                                                int n, double* __restrict__ out) {
+// One 1024-thread block; four independent loads in flight per thread, merged in index order.
+constexpr int kFinalizeThreads = 1024;
+__global__ __launch_bounds__(kFinalizeThreads) void finalize_pairs_kernel(const double2* __restrict__ partial,
+                                                                          int n, double* __restrict__ out) {
   double s = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) acc_merge(s, c, partial[i].x, partial[i].y);
-  block_reduce_pair(s, c);
+  constexpr int T = kFinalizeThreads;
+  int i = threadIdx.x;
+  for (; i + 3 * T < n; i += 4 * T) {
+    const double2 a = partial[i], b = partial[i + T], e = partial[i + 2 * T], f = partial[i + 3 * T];
+    acc_merge(s, c, a.x, a.y);
+    acc_merge(s, c, b.x, b.y);
+    acc_merge(s, c, e.x, e.y);
+    acc_merge(s, c, f.x, f.y);
+  }
+  for (; i < n; i += T) acc_merge(s, c, partial[i].x, partial[i].y);
+  block_reduce_pair<T>(s, c);
   if (threadIdx.x == 0) out[0] = s + c;
 }
 
@@ -176,7 +187,7 @@ FneCall make_call(const double* x, const double* dvals, const double* f, hipStre
 void fne_set_error(const std::string& msg) { g_last_error = msg; }
 
 int fne_finalize_energy(fne_plan* plan, int nblocks, double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kBlock), 0, stream, plan->partials, nblocks,
+  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kFinalizeThreads), 0, stream, plan->partials, nblocks,
                      out);
   FNE_LAUNCH_CHECK();
   return FNE_OK;
@@ -200,7 +211,14 @@ int fne_plan_create(const fne_plan_desc* desc, fne_plan** out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) == hipSuccess &&
       lds > 0)
     p->max_lds = size_t(lds);
-  p->n_partials = kEnergyBlocks;
+  {
+    // one partial per patch workgroup, per energy block (2*kBlock items each) or per fixed block
+    const long items = desc->n_elem > desc->n_free ? desc->n_elem : desc->n_free;
+    long need = (items + 2 * kBlock - 1) / (2 * kBlock);
+    if (desc->n_patch > need) need = desc->n_patch;
+    if (kEnergyBlocks > need) need = kEnergyBlocks;
+    p->n_partials = int(need);
+  }
   if (hipMalloc(&p->partials, sizeof(double2) * p->n_partials) != hipSuccess) {
     delete p;
     fne_set_error("hipMalloc of reduction scratch failed");
@@ -261,6 +279,21 @@ int fne_hessian(fne_plan* plan, const double* x, const double* dvals, const doub
   c.vals = vals;
   c.precision = precision;
   return dispatch(plan, kOpHessian, c);
+}
+
+int fne_energy_grad_hessian(fne_plan* plan, const double* x, const double* dvals, const double* f,
+                            double* out, double* g, void* vals, int32_t precision, fne_stream_t stream) {
+  if (!plan || !vals) return FNE_EINVAL;
+  if (precision != FNE_FP64 && precision != FNE_FP32) {
+    fne_set_error("precision must be FNE_FP64 or FNE_FP32");
+    return FNE_EINVAL;
+  }
+  FneCall c = make_call(x, dvals, f, (hipStream_t)stream);
+  c.out = out;
+  c.g = g;
+  c.vals = vals;
+  c.precision = precision;
+  return dispatch(plan, kOpFused, c);
 }
 
 int fne_element_eval(fne_plan* plan, const double* x, const double* dvals, double* e_out,
@@ -336,8 +369,8 @@ int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, n, a, b, partial);
   FNE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kBlock), 0, (hipStream_t)stream, partial, nb,
-                     out);
+  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kFinalizeThreads), 0, (hipStream_t)stream, partial,
+                     nb, out);
   FNE_LAUNCH_CHECK();
   return FNE_OK;
 }

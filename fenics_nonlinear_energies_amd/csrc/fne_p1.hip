@@ -20,23 +20,39 @@
 namespace {
 
 constexpr int kMaxQ = 16;
+// compile-time families: the p-Laplace exponent gets exact specialisations for p = 3 and p = 2
+constexpr int kFamPL = FNE_KIND_PLAPLACE;
+constexpr int kFamGL = FNE_KIND_GINZBURG_LANDAU;
+constexpr int kFamPL3 = 101;
+constexpr int kFamPL2 = 102;
+template <int fam>
+constexpr bool is_plaplace() {
+  return fam == kFamPL || fam == kFamPL3 || fam == kFamPL2;
+}
 
 struct P1Elem {
   double v[3];
   double dvx[3], dvy[3], vol;
 };
 
-__device__ __forceinline__ void p1_load(const fne_plan_desc& d, int e, const double* __restrict__ x,
-                                        const double* __restrict__ dvals, P1Elem& el) {
-  const int ne = d.n_elem;
+// element e of SoA arrays with stride ne: codes + geometry, then the gathered nodal values
+__device__ __forceinline__ void p1_load_raw(const int32_t* __restrict__ edof, const double* __restrict__ egeo,
+                                            long ne, long e, int c[3], P1Elem& el) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int c = d.edof[a * ne + e];
-    el.v[a] = gather_dof(c, d.n_free, x, dvals);
-    el.dvx[a] = d.egeo[a * ne + e];
-    el.dvy[a] = d.egeo[(3 + a) * ne + e];
+    c[a] = edof[a * ne + e];
+    el.dvx[a] = egeo[a * ne + e];
+    el.dvy[a] = egeo[(3 + a) * ne + e];
   }
-  el.vol = d.egeo[6 * ne + e];
+  el.vol = egeo[6 * ne + e];
+}
+
+__device__ __forceinline__ void p1_load(const fne_plan_desc& d, int e, const double* __restrict__ x,
+                                        const double* __restrict__ dvals, P1Elem& el) {
+  int c[3];
+  p1_load_raw(d.edof, d.egeo, d.n_elem, e, c, el);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) el.v[a] = gather_dof(c[a], d.n_free, x, dvals);
 }
 
 // s^(p/2) and friends; p==2 and p==3 take exact fast paths
@@ -46,8 +62,9 @@ __device__ __forceinline__ double pl_pow_half(double s, double p) {
   return pow(s, 0.5 * p);
 }
 
-__device__ __forceinline__ double p1_energy(int fam, const P1Elem& el, const double* prm,
-                                            const double* __restrict__ qd, int nq) {
+template <int fam>
+__device__ __forceinline__ double p1_energy(const P1Elem& el, const double* prm, const double* __restrict__ qd,
+                                            int nq) {
   double gx = 0.0, gy = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -55,7 +72,11 @@ __device__ __forceinline__ double p1_energy(int fam, const P1Elem& el, const dou
     gy += el.v[a] * el.dvy[a];
   }
   const double s = gx * gx + gy * gy;
-  if (fam == FNE_KIND_PLAPLACE) {
+  if constexpr (fam == kFamPL3) {
+    return (1.0 / 3.0) * (s * sqrt(s)) * el.vol;
+  } else if constexpr (fam == kFamPL2) {
+    return (1.0 / 2.0) * s * el.vol;
+  } else if constexpr (fam == kFamPL) {
     const double p = prm[0];
     return (1.0 / p) * pl_pow_half(s, p) * el.vol;
   }
@@ -70,9 +91,9 @@ __device__ __forceinline__ double p1_energy(int fam, const P1Elem& el, const dou
 }
 
 // gradient (3) and symmetric Hessian (6: 00 11 22 01 02 12), FP64
-__device__ __forceinline__ void p1_grad_hess(int fam, const P1Elem& el, const double* prm,
-                                             const double* __restrict__ qd, int nq, bool want_g,
-                                             bool want_h, double ge[3], double he[6]) {
+template <int fam>
+__device__ __forceinline__ void p1_grad_hess(const P1Elem& el, const double* prm, const double* __restrict__ qd,
+                                             int nq, bool want_g, bool want_h, double ge[3], double he[6]) {
   double gx = 0.0, gy = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -85,11 +106,18 @@ __device__ __forceinline__ void p1_grad_hess(int fam, const P1Elem& el, const do
   for (int a = 0; a < 3; ++a) t[a] = gx * el.dvx[a] + gy * el.dvy[a];
   const int ia[6] = {0, 1, 2, 0, 0, 1};
   const int ib[6] = {0, 1, 2, 1, 2, 2};
-  if (fam == FNE_KIND_PLAPLACE) {
-    const double p = prm[0];
+  if constexpr (is_plaplace<fam>()) {
+    const double p = fam == kFamPL3 ? 3.0 : (fam == kFamPL2 ? 2.0 : prm[0]);
     // c1 = s^(p/2-1), c2 = (p-2) s^(p/2-2)
     double c1, c2;
-    if (p == 2.0) {
+    if constexpr (fam == kFamPL2) {
+      c1 = 1.0;
+      c2 = 0.0;
+    } else if constexpr (fam == kFamPL3) {
+      const double r = sqrt(s);
+      c1 = r;
+      c2 = s > 0.0 ? 1.0 / r : 0.0;
+    } else if (p == 2.0) {
       c1 = 1.0;
       c2 = 0.0;
     } else if (p == 3.0) {
@@ -152,110 +180,311 @@ __device__ __forceinline__ int sym3(int a, int b) {
 // ---------------------------------------------------------------------------------------------
 // energy: grid-stride over elements (+ load term over free DOFs), compensated block partials
 // ---------------------------------------------------------------------------------------------
+template <int FAM>
 __global__ __launch_bounds__(kBlock) void p1_energy_kernel(fne_plan_desc d, const double* __restrict__ x,
                                                            const double* __restrict__ dvals,
                                                            const double* __restrict__ f,
                                                            double2* __restrict__ partial) {
+  // block b covers elements [b*2B, (b+1)*2B) and free DOFs [b*2B, (b+1)*2B) of the load term
   double s = 0.0, c = 0.0;
   const double* prm = d.param;
-  const int stride = gridDim.x * blockDim.x;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.n_elem; e += stride) {
-    P1Elem el;
-    p1_load(d, e, x, dvals, el);
-    double E = p1_energy(d.kind, el, prm, d.qdata, d.nq);
-    if (d.eweight) E *= d.eweight[e];
+  const int base = blockIdx.x * 2 * blockDim.x + threadIdx.x;
+  const int e1 = base, e2 = base + blockDim.x;
+  P1Elem el, el2;
+  if (e1 < d.n_elem) p1_load(d, e1, x, dvals, el);
+  if (e2 < d.n_elem) p1_load(d, e2, x, dvals, el2);
+  if (e1 < d.n_elem) {
+    double E = p1_energy<FAM>(el, prm, d.qdata, d.nq);
+    if (d.eweight) E *= d.eweight[e1];
+    acc_add(s, c, E);
+  }
+  if (e2 < d.n_elem) {
+    double E = p1_energy<FAM>(el2, prm, d.qdata, d.nq);
+    if (d.eweight) E *= d.eweight[e2];
     acc_add(s, c, E);
   }
   if (f) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_free; i += stride)
-      acc_add(s, c, -(f[i] * x[i]));
+    if (e1 < d.n_free) acc_add(s, c, -(f[e1] * x[e1]));
+    if (e2 < d.n_free) acc_add(s, c, -(f[e2] * x[e2]));
   }
   block_reduce_pair(s, c);
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s, c);
 }
 
 // ---------------------------------------------------------------------------------------------
-// owner-computes patch assembly.  One workgroup per patch of <= blockDim consecutive rows.
-// Phase A: the patch's elements (owned + halo) are evaluated, chunk by chunk, into LDS.
-// Phase B: each row thread walks its incidence list (sorted by element slot) and accumulates
-//          its gradient entry and its CSR row slice in a fixed order -> deterministic, no atomics.
-// Incidence item: slot[0:13) | a[13:15) | pos_b[15+5b : 20+5b) (31 = constrained column).
+// Colored patch assembly (FNE_FLAG_COLORED).  One workgroup per patch of <= blockDim consecutive
+// rows; the patch's elements (owned + halo, stored patch-major) are evaluated once, EPT per
+// thread, into registers.  Then, color by color, each element adds its contributions to the
+// patch's OWNED rows directly into the LDS copy of the CSR slice (and gradient slice).  Elements
+// of one color share no owned row, so the adds never race; the order (color, then slot) is
+// fixed, so the result is bitwise reproducible.  No element staging, no incidence lists.
+// item (uint64): pos_ab[5(3a+b) : +5) (31 = none) | energy owner [48] | color [52:60)
 // ---------------------------------------------------------------------------------------------
-template <typename HT, bool WANT_G, bool WANT_H>
-__global__ __launch_bounds__(kBlock) void p1_patch_kernel(fne_plan_desc d, const double* __restrict__ x,
+template <int FAM, typename HT, bool WANT_E, bool WANT_G, bool WANT_H, int EPT>
+__global__ __launch_bounds__(kBlock) void p1_color_kernel(fne_plan_desc d, const double* __restrict__ x,
                                                           const double* __restrict__ dvals,
                                                           const double* __restrict__ f,
-                                                          double* __restrict__ g, HT* __restrict__ vals) {
+                                                          double* __restrict__ g, HT* __restrict__ vals,
+                                                          double2* __restrict__ partial) {
   extern __shared__ double smem[];
   const int p = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int chunk = d.chunk;
+  const int B = blockDim.x;
   const int r0 = d.prow[p], r1 = d.prow[p + 1];
+  const int nrows = r1 - r0;
   const int k0 = d.indptr[r0];
   const int nslice = WANT_H ? d.indptr[r1] - k0 : 0;
-  const int eb = d.pelem_ptr[p];
-  const int ne = d.pelem_ptr[p + 1] - eb;
+  const long eb = d.pelem_ptr[p];
+  const int ne = d.pelem_ptr[p + 1] - int(eb);
+  const int ncol = d.pcolors[p];
 
-  double* sVal = smem;                       // [max_patch_nnz]
-  double* sH = sVal + (WANT_H ? d.max_patch_nnz : 0);  // [6][chunk]
-  double* sG = sH + (WANT_H ? 6 * chunk : 0);           // [3][chunk]
+  double* sVal = smem;                                 // [max_patch_nnz]
+  double* sG = sVal + (WANT_H ? d.max_patch_nnz : 0);  // [B]
+  int* sPtr = reinterpret_cast<int*>(sG + (WANT_G ? B : 0));  // [B + 1]
+  if (WANT_H) {
+    for (int k = tid; k < nslice; k += B) sVal[k] = 0.0;
+    for (int i = tid; i <= nrows; i += B) sPtr[i] = d.indptr[r0 + i] - k0;
+  }
+  if (WANT_G)
+    for (int i = tid; i < nrows; i += B) sG[i] = 0.0;
 
+  double es = 0.0, ec = 0.0;
+  const double* prm = d.param;
+  P1Elem el[EPT];
+  int code[EPT][3];
+  uint64_t item[EPT];
+  bool act[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int sl = tid + k * B;
+    act[k] = sl < ne;
+    if (act[k]) {
+      p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code[k], el[k]);
+      item[k] = d.pitem[eb + sl];
+    }
+  }
+  double ge[EPT][3], he[EPT][6];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    if (!act[k]) continue;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) el[k].v[a] = gather_dof(code[k][a], d.n_free, x, dvals);
+    if (WANT_E && ((item[k] >> 48) & 1ull)) acc_add(es, ec, p1_energy<FAM>(el[k], prm, d.qdata, d.nq));
+    if (WANT_G || WANT_H) p1_grad_hess<FAM>(el[k], prm, d.qdata, d.nq, WANT_G, WANT_H, ge[k], he[k]);
+  }
+  __syncthreads();
+  if (WANT_G || WANT_H) {
+    for (int c = 0; c < ncol; ++c) {
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        if (!act[k] || int((item[k] >> 52) & 0xFFull) != c) continue;
+        // the (up to) 9 targets of one element are distinct LDS words: issue every read, then
+        // every write, instead of 9 dependent read-add-write round trips
+        int rl[3], addr[9];
+        double oldg[3], oldh[9];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          rl[a] = code[k][a] - r0;
+          const bool own = rl[a] >= 0 && rl[a] < nrows;  // else: another patch's (or constrained) row
+          if (!own) rl[a] = -1;
+          if (WANT_H) {
+            const int off = own ? sPtr[rl[a]] : 0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+              const int pos = int((item[k] >> (5 * (3 * a + b))) & 31ull);
+              addr[3 * a + b] = (own && pos != 31) ? off + pos : -1;
+            }
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (WANT_G && rl[a] >= 0) oldg[a] = sG[rl[a]];
+        if (WANT_H) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q)
+            if (addr[q] >= 0) oldh[q] = sVal[addr[q]];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          if (WANT_G && rl[a] >= 0) sG[rl[a]] = oldg[a] + ge[k][a];
+        if (WANT_H) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q)
+            if (addr[q] >= 0) sVal[addr[q]] = oldh[q] + he[k][sym3(q / 3, q % 3)];
+        }
+      }
+      __syncthreads();
+    }
+  }
   if (WANT_H)
-    for (int k = tid; k < nslice; k += blockDim.x) sVal[k] = 0.0;
+    for (int k = tid; k < nslice; k += B) vals[k0 + k] = HT(sVal[k]);
+  if (WANT_G)
+    for (int i = tid; i < nrows; i += B) g[r0 + i] = f ? sG[i] - f[r0 + i] : sG[i];
+  if (WANT_E) {
+    block_reduce_pair(es, ec);
+    if (tid == 0) partial[p] = make_double2(es, ec);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Staged patch assembly (FNE_FLAG_STAGED).  Phase A: the patch's elements (patch-major, owned
+// first) are evaluated one per thread per iteration; their 6 Hessian values (type HT) and 3
+// gradient values (FP64) are staged in LDS ([value][slot], conflict-free writes).  Phase B:
+// thread t owns row r0+t and walks its incidence list (prefetched before phase A); the three
+// columns of one incidence are distinct, so their read-modify-writes into the row's part of an
+// LDS copy of the CSR slice are issued together.  Every slice word is touched by one thread in a
+// fixed order (bitwise reproducible, no atomics); the slice then leaves as coalesced stores.
+// HT = float accumulates in FP32 (FP32 Hessian mode), HT = double in FP64.
+// ---------------------------------------------------------------------------------------------
+constexpr int kStagedPrefetch = 8;
+
+template <int FAM, typename HT, bool WANT_E, bool WANT_G, bool WANT_H>
+__global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, const double* __restrict__ x,
+                                                           const double* __restrict__ dvals,
+                                                           const double* __restrict__ f,
+                                                           double* __restrict__ g, HT* __restrict__ vals,
+                                                           double2* __restrict__ partial) {
+  extern __shared__ double smem[];
+  const int p = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int cap = d.chunk;
+  const int r0 = d.prow[p], r1 = d.prow[p + 1];
+  const long eb = d.pelem_ptr[p];
+  const int ne = d.pelem_ptr[p + 1] - int(eb);
+  const int nown = WANT_E ? d.pown[p] : 0;
+  const int k0 = WANT_H ? d.indptr[r0] : 0;
+  const int nslice = WANT_H ? d.indptr[r1] - k0 : 0;
+  // LDS: [gradient FP64: 3*cap] [slice HT: max_patch_nnz (8-aligned)] [Hessian HT: 6*cap]
+  double* sG = smem;
+  HT* sVal = reinterpret_cast<HT*>(smem + (WANT_G ? 3 * cap : 0));
+  HT* sH = sVal + (WANT_H ? ((d.max_patch_nnz + 1) & ~1) : 0);
 
   const int r = r0 + tid;
   const bool has_row = r < r1;
-  int ip = has_row ? d.inc_ptr[r] : 0;
-  const int ie = has_row ? d.inc_ptr[r + 1] : 0;
-  const int roff = has_row ? d.indptr[r] - k0 : 0;
-  double gacc = 0.0;
-  const double* prm = d.param;
-  __syncthreads();
+  const int ib = has_row ? d.inc_ptr[r] : 0;
+  const int cnt = has_row ? d.inc_ptr[r + 1] - ib : 0;
+  const int roff = (WANT_H && has_row) ? d.indptr[r] - k0 : 0;
+  uint32_t items[kStagedPrefetch];
+#pragma unroll
+  for (int k = 0; k < kStagedPrefetch; ++k) items[k] = (k < cnt) ? d.inc[ib + k] : 0u;
+  if (WANT_H)
+    for (int k = tid; k < nslice; k += B) sVal[k] = HT(0);
 
-  for (int c0 = 0; c0 < ne; c0 += chunk) {
-    const int nc = min(chunk, ne - c0);
-    for (int sl = tid; sl < nc; sl += blockDim.x) {
-      const int e = d.pelem[eb + c0 + sl];
-      P1Elem el;
-      p1_load(d, e, x, dvals, el);
+  double es = 0.0, ec = 0.0;
+  const double* prm = d.param;
+#pragma unroll 1
+  for (int sl = tid; sl < ne; sl += B) {
+    P1Elem el;
+    int code[3];
+    p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code, el);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) el.v[a] = gather_dof(code[a], d.n_free, x, dvals);
+    if (WANT_E && sl < nown) acc_add(es, ec, p1_energy<FAM>(el, prm, d.qdata, d.nq));
+    if (WANT_G || WANT_H) {
       double ge[3], he[6];
-      p1_grad_hess(d.kind, el, prm, d.qdata, d.nq, WANT_G, WANT_H, ge, he);
+      p1_grad_hess<FAM>(el, prm, d.qdata, d.nq, WANT_G, WANT_H, ge, he);
       if (WANT_H) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) sH[k * chunk + sl] = he[k];
+        for (int q = 0; q < 6; ++q) sH[q * cap + sl] = HT(he[q]);
       }
       if (WANT_G) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) sG[a * chunk + sl] = ge[a];
+        for (int a = 0; a < 3; ++a) sG[a * cap + sl] = ge[a];
       }
     }
+  }
+  if (WANT_G || WANT_H) {
     __syncthreads();
     if (has_row) {
-      while (ip < ie) {
-        const uint32_t it = d.inc[ip];
-        const int slot = int(it & 0x1FFFu);
-        if (slot >= c0 + nc) break;
-        const int sl = slot - c0;
+      double gacc = 0.0;
+      auto consume = [&](uint32_t it) {
+        const int s = int(it & 0x1FFFu);
         const int a = int((it >> 13) & 3u);
-        if (WANT_G) gacc += sG[a * chunk + sl];
+        if (WANT_G) gacc += sG[a * cap + s];
         if (WANT_H) {
+          int pos[3];
+          HT h[3], old[3];
 #pragma unroll
           for (int b = 0; b < 3; ++b) {
-            const int pos = int((it >> (15 + 5 * b)) & 31u);
-            if (pos != 31) sVal[roff + pos] += sH[sym3(a, b) * chunk + sl];
+            pos[b] = int((it >> (15 + 5 * b)) & 31u);
+            h[b] = sH[sym3(a, b) * cap + s];
           }
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+            if (pos[b] != 31) old[b] = sVal[roff + pos[b]];
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+            if (pos[b] != 31) sVal[roff + pos[b]] = old[b] + h[b];
         }
-        ++ip;
+      };
+#pragma unroll
+      for (int k = 0; k < kStagedPrefetch; ++k)
+        if (k < cnt) consume(items[k]);
+      for (int k = kStagedPrefetch; k < cnt; ++k) consume(d.inc[ib + k]);
+      if (WANT_G) g[r] = f ? gacc - f[r] : gacc;
+    }
+    if (WANT_H) {
+      __syncthreads();
+      for (int k = tid; k < nslice; k += B) vals[k0 + k] = sVal[k];
+    }
+  }
+  if (WANT_E) {
+    block_reduce_pair(es, ec);
+    if (tid == 0) partial[p] = make_double2(es, ec);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row-gather assembly (FNE_FLAG_ROW_GATHER): one thread per CSR row, no barriers, no staging.
+// Each incident element is re-evaluated by each of its vertex rows (P1 element math is cheap;
+// the re-reads hit L1/L2 because neighbouring rows share elements).  Row accumulators live in
+// LDS, column-major per thread ([pos][thread]) so that dynamic positions stay conflict-free.
+// item (uint64): e[0:32) | a[32:34) | pos_b[40+8b : 48+8b)  (255 = constrained column)
+// ---------------------------------------------------------------------------------------------
+template <int FAM, typename HT, bool WANT_G, bool WANT_H>
+__global__ __launch_bounds__(kBlock) void p1_rowgather_kernel(fne_plan_desc d, const double* __restrict__ x,
+                                                              const double* __restrict__ dvals,
+                                                              const double* __restrict__ f,
+                                                              double* __restrict__ g, HT* __restrict__ vals) {
+  extern __shared__ double sAcc[];  // [max_row_len][blockDim]
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const bool has_row = r < d.n_free;
+  const int kb = has_row ? d.indptr[r] : 0;
+  const int len = has_row ? d.indptr[r + 1] - kb : 0;
+  if (WANT_H)
+    for (int j = 0; j < len; ++j) sAcc[j * B + tid] = 0.0;
+  if (!has_row) return;
+  const uint64_t* inc = reinterpret_cast<const uint64_t*>(d.inc);
+  const int ib = d.inc_ptr[r], ie = d.inc_ptr[r + 1];
+  double gacc = 0.0;
+  const double* prm = d.param;
+  for (int it = ib; it < ie; ++it) {
+    const uint64_t item = inc[it];
+    const int e = int(item & 0xFFFFFFFFull);
+    const int a = int((item >> 32) & 3ull);
+    P1Elem el;
+    p1_load(d, e, x, dvals, el);
+    double ge[3], he[6];
+    p1_grad_hess<FAM>(el, prm, d.qdata, d.nq, WANT_G, WANT_H, ge, he);
+    if (WANT_G) gacc += ge[a];
+    if (WANT_H) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const int pos = int((item >> (40 + 8 * b)) & 0xFFull);
+        if (pos != 255) sAcc[pos * B + tid] += he[sym3(a, b)];
       }
     }
-    __syncthreads();
   }
   if (WANT_H)
-    for (int k = tid; k < nslice; k += blockDim.x) vals[k0 + k] = HT(sVal[k]);
-  if (WANT_G && has_row) g[r] = f ? gacc - f[r] : gacc;
+    for (int j = 0; j < len; ++j) vals[kb + j] = HT(sAcc[j * B + tid]);
+  if (WANT_G) g[r] = f ? gacc - f[r] : gacc;
 }
 
 // dense per-element outputs (tests, autograd boundary)
+template <int FAM>
 __global__ __launch_bounds__(kBlock) void p1_element_kernel(fne_plan_desc d, const double* __restrict__ x,
                                                             const double* __restrict__ dvals,
                                                             double* __restrict__ e_out,
@@ -266,10 +495,10 @@ __global__ __launch_bounds__(kBlock) void p1_element_kernel(fne_plan_desc d, con
   P1Elem el;
   p1_load(d, e, x, dvals, el);
   const int ne = d.n_elem;
-  if (e_out) e_out[e] = p1_energy(d.kind, el, d.param, d.qdata, d.nq);
+  if (e_out) e_out[e] = p1_energy<FAM>(el, d.param, d.qdata, d.nq);
   if (ge_out || he_out) {
     double ge[3], he[6];
-    p1_grad_hess(d.kind, el, d.param, d.qdata, d.nq, ge_out != nullptr, he_out != nullptr, ge, he);
+    p1_grad_hess<FAM>(el, d.param, d.qdata, d.nq, ge_out != nullptr, he_out != nullptr, ge, he);
     if (ge_out)
       for (int a = 0; a < 3; ++a) ge_out[a * ne + e] = ge[a];
     if (he_out)
@@ -280,15 +509,22 @@ __global__ __launch_bounds__(kBlock) void p1_element_kernel(fne_plan_desc d, con
 
 }  // namespace
 
-int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
+template <int FAM>
+int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
-  if (d.kind == FNE_KIND_GINZBURG_LANDAU && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
+  if (FAM == kFamGL && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
     fne_set_error("Ginzburg-Landau plan needs 1..16 quadrature points in qdata");
     return FNE_EINVAL;
   }
   if (op == kOpEnergy) {
-    const int nb = plan->n_partials;
-    hipLaunchKernelGGL(p1_energy_kernel, dim3(nb), dim3(kBlock), 0, c.stream, d, c.x, c.dvals, c.f,
+    const int items = d.n_elem > d.n_free ? d.n_elem : d.n_free;
+    int nb = (items + 2 * kBlock - 1) / (2 * kBlock);
+    if (nb < 1) nb = 1;
+    if (nb > plan->n_partials) {
+      fne_set_error("energy reduction scratch too small");
+      return FNE_EINVAL;
+    }
+    hipLaunchKernelGGL(p1_energy_kernel<FAM>, dim3(nb), dim3(kBlock), 0, c.stream, d, c.x, c.dvals, c.f,
                        plan->partials);
     FNE_LAUNCH_CHECK();
     return fne_finalize_energy(plan, nb, c.out, c.stream);
@@ -296,46 +532,167 @@ int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   if (op == kOpElementEval) {
     const int nb = (d.n_elem + kBlock - 1) / kBlock;
     if (nb > 0) {
-      hipLaunchKernelGGL(p1_element_kernel, dim3(nb), dim3(kBlock), 0, c.stream, d, c.x, c.dvals,
+      hipLaunchKernelGGL(p1_element_kernel<FAM>, dim3(nb), dim3(kBlock), 0, c.stream, d, c.x, c.dvals,
                          c.e_out, c.ge_out, c.he_out);
       FNE_LAUNCH_CHECK();
     }
     return FNE_OK;
   }
-  if (d.n_patch <= 0) return FNE_OK;
   const bool want_h = (op == kOpHessian);
   const bool want_g = (op == kOpGradient) || (c.g != nullptr);
-  size_t shm = 0;
-  if (want_h) shm += sizeof(double) * (size_t(d.max_patch_nnz) + 6 * size_t(d.chunk));
-  if (want_g) shm += sizeof(double) * 3 * size_t(d.chunk);
-  if (shm > plan->max_lds) {
-    fne_set_error("patch LDS request " + std::to_string(shm) + " B exceeds the device limit " +
-                  std::to_string(plan->max_lds) + " B; rebuild the plan with a smaller chunk");
+  if (d.flags & FNE_FLAG_ROW_GATHER) {
+    if (op == kOpFused) {
+      if (c.out) {
+        const int rc = launch_p1<FAM>(plan, kOpEnergy, c);
+        if (rc) return rc;
+      }
+      return launch_p1<FAM>(plan, kOpHessian, c);
+    }
+    if (d.n_free <= 0) return FNE_OK;
+    const size_t shm_rg = want_h ? sizeof(double) * size_t(d.max_row_len) * kBlock : 0;
+    if (shm_rg > plan->max_lds) {
+      fne_set_error("row-gather LDS request exceeds the device limit");
+      return FNE_EINVAL;
+    }
+    const dim3 grid((d.n_free + kBlock - 1) / kBlock), block(kBlock);
+    if (!want_h)
+      hipLaunchKernelGGL((p1_rowgather_kernel<FAM, double, true, false>), grid, block, 0, c.stream, d, c.x, c.dvals,
+                         c.f, c.g, (double*)nullptr);
+    else if (c.precision == FNE_FP32) {
+      if (want_g)
+        hipLaunchKernelGGL((p1_rowgather_kernel<FAM, float, true, true>), grid, block, shm_rg, c.stream, d, c.x,
+                           c.dvals, c.f, c.g, (float*)c.vals);
+      else
+        hipLaunchKernelGGL((p1_rowgather_kernel<FAM, float, false, true>), grid, block, shm_rg, c.stream, d, c.x,
+                           c.dvals, c.f, c.g, (float*)c.vals);
+    } else {
+      if (want_g)
+        hipLaunchKernelGGL((p1_rowgather_kernel<FAM, double, true, true>), grid, block, shm_rg, c.stream, d, c.x,
+                           c.dvals, c.f, c.g, (double*)c.vals);
+      else
+        hipLaunchKernelGGL((p1_rowgather_kernel<FAM, double, false, true>), grid, block, shm_rg, c.stream, d, c.x,
+                           c.dvals, c.f, c.g, (double*)c.vals);
+    }
+    FNE_LAUNCH_CHECK();
+    return FNE_OK;
+  }
+  if (d.n_patch <= 0) {
+    if (op == kOpFused && c.out) return launch_p1<FAM>(plan, kOpEnergy, c);
+    return FNE_OK;
+  }
+  const bool want_e = (op == kOpFused) && (c.out != nullptr);
+  if (d.flags & FNE_FLAG_STAGED) {
+    if (op == kOpFused && c.vals == nullptr) {
+      fne_set_error("fused assembly needs the CSR value buffer");
+      return FNE_EINVAL;
+    }
+    const bool sh = want_h || op == kOpFused;
+    const bool sg = want_g || (op == kOpFused && c.g != nullptr);
+    if (!d.pedof || !d.pgeo || !d.pown || !d.inc || !d.inc_ptr) {
+      fne_set_error("staged patch arrays missing from the plan");
+      return FNE_EINVAL;
+    }
+    const size_t hsz = c.precision == FNE_FP32 ? 4 : 8;
+    size_t shm = 0;
+    if (sg) shm += sizeof(double) * 3 * size_t(d.chunk);
+    if (sh) shm += hsz * (size_t((d.max_patch_nnz + 1) & ~1) + 6 * size_t(d.chunk));
+    if (shm > plan->max_lds) {
+      fne_set_error("staged patch LDS request exceeds the device limit");
+      return FNE_EINVAL;
+    }
+    if (d.max_row_len > 30) {
+      fne_set_error("P1 staged kernel supports CSR rows of at most 30 entries");
+      return FNE_EINVAL;
+    }
+    const dim3 grid(d.n_patch), block(kBlock);
+    double2* part = plan->partials;
+#define ST_GO(HT, E, G, H)                                                                              \
+  hipLaunchKernelGGL((p1_staged_kernel<FAM, HT, E, G, H>), grid, block, shm, c.stream, d, c.x, c.dvals, c.f, c.g, \
+                     (HT*)c.vals, part)
+#define ST_MR(HT, E, G, H) ST_GO(HT, E, G, H)
+    if (!sh) {
+      ST_MR(double, false, true, false);
+    } else if (c.precision == FNE_FP32) {
+      if (want_e && sg) ST_MR(float, true, true, true);
+      else if (want_e) ST_MR(float, true, false, true);
+      else if (sg) ST_MR(float, false, true, true);
+      else ST_MR(float, false, false, true);
+    } else {
+      if (want_e && sg) ST_MR(double, true, true, true);
+      else if (want_e) ST_MR(double, true, false, true);
+      else if (sg) ST_MR(double, false, true, true);
+      else ST_MR(double, false, false, true);
+    }
+#undef ST_MR
+#undef ST_GO
+    FNE_LAUNCH_CHECK();
+    if (want_e) return fne_finalize_energy(plan, d.n_patch, c.out, c.stream);
+    return FNE_OK;
+  }
+  if (!(d.flags & FNE_FLAG_COLORED)) {
+    fne_set_error("P1 plan has none of FNE_FLAG_COLORED, FNE_FLAG_STAGED, FNE_FLAG_ROW_GATHER");
     return FNE_EINVAL;
   }
-  if (d.max_row_len > 30) {
-    fne_set_error("P1 patch kernel supports CSR rows of at most 30 entries");
+  if (op == kOpFused && c.vals == nullptr) {
+    fne_set_error("fused assembly needs the CSR value buffer");
+    return FNE_EINVAL;
+  }
+  const bool want_hh = want_h || op == kOpFused;
+  const bool want_gg = want_g || (op == kOpFused && c.g != nullptr);
+  if (!d.pedof || !d.pgeo || !d.pitem || !d.pcolors) {
+    fne_set_error("colored patch arrays missing from the plan");
+    return FNE_EINVAL;
+  }
+  const int ept = d.chunk;  // elements per thread (patch elements <= ept * blockDim)
+  if (ept < 1 || ept > 3) {
+    fne_set_error("colored P1 plans support 1..3 elements per thread");
+    return FNE_EINVAL;
+  }
+  size_t shm = 0;
+  if (want_hh) shm += sizeof(double) * size_t(d.max_patch_nnz) + sizeof(int) * (kBlock + 1);
+  if (want_gg) shm += sizeof(double) * kBlock;
+  if (shm > plan->max_lds) {
+    fne_set_error("patch LDS request " + std::to_string(shm) + " B exceeds the device limit");
     return FNE_EINVAL;
   }
   const dim3 grid(d.n_patch), block(kBlock);
-  if (!want_h) {
-    hipLaunchKernelGGL((p1_patch_kernel<double, true, false>), grid, block, shm, c.stream, d, c.x,
-                       c.dvals, c.f, c.g, (double*)nullptr);
+  double2* part = plan->partials;
+#define P1_GO(HT, E, G, H, EP)                                                                          \
+  hipLaunchKernelGGL((p1_color_kernel<FAM, HT, E, G, H, EP>), grid, block, shm, c.stream, d, c.x, c.dvals, c.f, \
+                     c.g, (HT*)c.vals, part)
+#define P1_EPT(HT, E, G, H)          \
+  do {                               \
+    if (ept == 1) P1_GO(HT, E, G, H, 1); \
+    else if (ept == 2) P1_GO(HT, E, G, H, 2); \
+    else P1_GO(HT, E, G, H, 3);      \
+  } while (0)
+  if (!want_hh) {
+    P1_EPT(double, false, true, false);
   } else if (c.precision == FNE_FP32) {
-    if (want_g)
-      hipLaunchKernelGGL((p1_patch_kernel<float, true, true>), grid, block, shm, c.stream, d, c.x,
-                         c.dvals, c.f, c.g, (float*)c.vals);
-    else
-      hipLaunchKernelGGL((p1_patch_kernel<float, false, true>), grid, block, shm, c.stream, d, c.x,
-                         c.dvals, c.f, c.g, (float*)c.vals);
+    if (want_e && want_gg) P1_EPT(float, true, true, true);
+    else if (want_e) P1_EPT(float, true, false, true);
+    else if (want_gg) P1_EPT(float, false, true, true);
+    else P1_EPT(float, false, false, true);
   } else {
-    if (want_g)
-      hipLaunchKernelGGL((p1_patch_kernel<double, true, true>), grid, block, shm, c.stream, d, c.x,
-                         c.dvals, c.f, c.g, (double*)c.vals);
-    else
-      hipLaunchKernelGGL((p1_patch_kernel<double, false, true>), grid, block, shm, c.stream, d, c.x,
-                         c.dvals, c.f, c.g, (double*)c.vals);
+    if (want_e && want_gg) P1_EPT(double, true, true, true);
+    else if (want_e) P1_EPT(double, true, false, true);
+    else if (want_gg) P1_EPT(double, false, true, true);
+    else P1_EPT(double, false, false, true);
   }
+#undef P1_EPT
+#undef P1_GO
+  FNE_LAUNCH_CHECK();
+  if (want_e) return fne_finalize_energy(plan, d.n_patch, c.out, c.stream);
   FNE_LAUNCH_CHECK();
   return FNE_OK;
+}
+
+int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
+  if (plan->d.kind == FNE_KIND_PLAPLACE) {
+    const double p = plan->d.param[0];
+    if (p == 3.0) return launch_p1<kFamPL3>(plan, op, c);
+    if (p == 2.0) return launch_p1<kFamPL2>(plan, op, c);
+    return launch_p1<kFamPL>(plan, op, c);
+  }
+  return launch_p1<kFamGL>(plan, op, c);
 }

@@ -278,6 +278,13 @@ int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c) {
     FNE_LAUNCH_CHECK();
     return fne_finalize_energy(plan, nb, c.out, c.stream);
   }
+  if (op == kOpFused) {
+    if (c.out) {
+      const int rc = fne_launch_tet(plan, kOpEnergy, c);
+      if (rc) return rc;
+    }
+    return fne_launch_tet(plan, kOpHessian, c);
+  }
   if (op == kOpElementEval) {
     const int nb = (d.n_elem + kBlock - 1) / kBlock;
     if (nb > 0) {

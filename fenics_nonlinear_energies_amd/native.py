@@ -21,6 +21,9 @@ FNE_KIND_MOHR_COULOMB = 4
 FNE_FP64 = 0
 FNE_FP32 = 1
 FNE_FLAG_ABS_DET = 1
+FNE_FLAG_ROW_GATHER = 2
+FNE_FLAG_COLORED = 4
+FNE_FLAG_STAGED = 8
 
 _i32p = C.POINTER(C.c_int32)
 _u32p = C.POINTER(C.c_uint32)
@@ -53,6 +56,12 @@ class FnePlanDesc(C.Structure):
         ("inc_ptr", C.c_void_p),
         ("inc", C.c_void_p),
         ("inc_aux", C.c_void_p),
+        ("n_pe", C.c_int64),
+        ("pedof", C.c_void_p),
+        ("pgeo", C.c_void_p),
+        ("pown", C.c_void_p),
+        ("pitem", C.c_void_p),
+        ("pcolors", C.c_void_p),
         ("param", C.c_double * 8),
         ("qdata", C.c_void_p),
         ("nq", C.c_int32),
@@ -74,6 +83,11 @@ _SIGNATURES = [
         [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
     ),
     (
+        "fne_energy_grad_hessian",
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
+    ),
+    (
         "fne_element_eval",
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -89,6 +103,11 @@ _SIGNATURES = [
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
     ),
     ("fne_dot", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    (
+        "fne_color_patch_elements",
+        C.c_int,
+        [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p],
+    ),
     ("fne_last_error", C.c_char_p, []),
     ("fne_version", C.c_char_p, []),
 ]

@@ -40,6 +40,11 @@ class PatchPlan:
     inc_aux: np.ndarray | None
     max_patch_nnz: int
     max_row_len: int
+    pown: np.ndarray | None = None  # int32 [n_patch] owned elements per patch (listed first)
+
+    @property
+    def max_patch_elems(self) -> int:
+        return int(np.diff(self.pelem_ptr).max(initial=0))
 
     @property
     def halo_factor(self) -> float:
@@ -62,7 +67,13 @@ def _elements_by_patch(row_of: np.ndarray, rows_per_patch: int, n_patch: int):
     valid = row_of >= 0
     masked = np.where(valid, row_of, np.iinfo(np.int64).max)
     min_row = masked.min(axis=1) if n_elem else np.zeros(0, dtype=np.int64)
-    owner = np.where(min_row < np.iinfo(np.int64).max, min_row // rows_per_patch, n_patch)
+    free_any = min_row < np.iinfo(np.int64).max
+    owner = np.where(free_any, min_row // rows_per_patch, n_patch)
+    # elements with only constrained DOFs still carry energy: spread them over the patches as
+    # owned elements without incidences (they add to the fused energy only)
+    dir_only = np.nonzero(~free_any)[0]
+    if n_patch > 0 and dir_only.size:
+        owner[dir_only] = np.arange(dir_only.size) % n_patch
     elem_order = np.argsort(owner, kind="stable").astype(np.int64)
     new_of_old = np.empty(n_elem, dtype=np.int64)
     new_of_old[elem_order] = np.arange(n_elem, dtype=np.int64)
@@ -71,7 +82,8 @@ def _elements_by_patch(row_of: np.ndarray, rows_per_patch: int, n_patch: int):
     valid_new = row_new >= 0
     e_idx = np.broadcast_to(np.arange(n_elem, dtype=np.int64)[:, None], row_new.shape)[valid_new]
     p_idx = row_new[valid_new] // rows_per_patch
-    pair_key = np.unique(p_idx * np.int64(n_elem) + e_idx)
+    lone = np.nonzero(~valid_new.any(axis=1) & (owner_new < n_patch))[0]
+    pair_key = np.unique(np.concatenate((p_idx * np.int64(n_elem) + e_idx, owner_new[lone] * np.int64(n_elem) + lone)))
     pp = pair_key // n_elem
     pe = pair_key % n_elem
     halo = (owner_new[pe] != pp).astype(np.int64)
@@ -81,7 +93,8 @@ def _elements_by_patch(row_of: np.ndarray, rows_per_patch: int, n_patch: int):
     pelem_ptr[1:] = np.cumsum(np.bincount(pp, minlength=n_patch))
     slot_sorted = np.empty(pair_key.size, dtype=np.int64)
     slot_sorted[order] = np.arange(pair_key.size, dtype=np.int64) - pelem_ptr[pp[order]]
-    return elem_order, new_of_old, row_new, pelem, pelem_ptr, pair_key, slot_sorted
+    pown = np.bincount(pp[halo == 0], minlength=n_patch)
+    return elem_order, new_of_old, row_new, pelem, pelem_ptr, pair_key, slot_sorted, pown
 
 
 def build_p1_plan(
@@ -102,7 +115,7 @@ def build_p1_plan(
     indices = np.asarray(indices, dtype=np.int64)
     prow = _patch_rows(n_free, rows_per_patch)
     n_patch = prow.size - 1
-    elem_order, _, ef, pelem, pelem_ptr, pair_key, slot_sorted = _elements_by_patch(
+    elem_order, _, ef, pelem, pelem_ptr, pair_key, slot_sorted, pown = _elements_by_patch(
         elem_free, rows_per_patch, n_patch
     )
     counts = np.diff(pelem_ptr)
@@ -147,6 +160,7 @@ def build_p1_plan(
         inc_aux=None,
         max_patch_nnz=int(patch_nnz.max()) if patch_nnz.size else 0,
         max_row_len=max_row_len,
+        pown=pown.astype(np.int32),
     )
 
 
@@ -206,7 +220,7 @@ def build_block_plan(
 
     prow_nodes = _patch_rows(n_nrows, nodes_per_patch)
     n_patch = prow_nodes.size - 1
-    elem_order, _, nr, pelem, pelem_ptr, pair_key, slot_sorted = _elements_by_patch(nrow, nodes_per_patch, n_patch)
+    elem_order, _, nr, pelem, pelem_ptr, pair_key, slot_sorted, pown = _elements_by_patch(nrow, nodes_per_patch, n_patch)
     if np.diff(pelem_ptr).max(initial=0) >= (1 << BLOCK_SLOT_BITS):
         raise ValueError("too many elements in one patch")
     e_i, a_i = np.nonzero(nr >= 0)
@@ -246,4 +260,166 @@ def build_block_plan(
         inc_aux=None,
         max_patch_nnz=int(patch_nnz.max()) if patch_nnz.size else 0,
         max_row_len=int(np.diff(indptr).max(initial=0)),
+        pown=pown.astype(np.int32),
     )
+
+
+def build_p1_rowgather_plan(
+    elem_free: np.ndarray,
+    indptr: np.ndarray,
+    indices: np.ndarray,
+    n_free: int,
+    *,
+    locality_rows: int = 256,
+) -> PatchPlan:
+    """Row-gather schedule for P1 scalar elements (one thread per CSR row, no staging).
+
+    Items (uint64): e[0:32) | a[32:34) | pos_b[40+8b : 48+8b) with 255 for a constrained
+    column; each row's items are sorted by element.  Elements are renumbered by the
+    ``locality_rows`` block of their first row so that neighbouring rows share cache lines.
+    """
+    elem_free = np.asarray(elem_free, dtype=np.int64)
+    n_elem = elem_free.shape[0]
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    n_blk = max(1, -(-n_free // locality_rows))
+    elem_order, _, ef, _, _, _, _, _ = _elements_by_patch(elem_free, locality_rows, n_blk)
+    row_len = np.diff(indptr)
+    if int(row_len.max(initial=0)) >= 255:
+        raise ValueError("row-gather items hold 8-bit column positions")
+    e_i, a_i = np.nonzero(ef >= 0)
+    r_i = ef[e_i, a_i]
+    csr_keys = np.repeat(np.arange(n_free, dtype=np.int64), row_len) * np.int64(n_free) + indices
+    item = e_i.astype(np.uint64) | (a_i.astype(np.uint64) << np.uint64(32))
+    for b in range(3):
+        c = ef[e_i, b]
+        pos = np.full(e_i.size, 255, dtype=np.int64)
+        ok = c >= 0
+        key = r_i[ok] * np.int64(n_free) + c[ok]
+        k = np.searchsorted(csr_keys, key)
+        if not np.array_equal(csr_keys[k], key):
+            raise RuntimeError("element coupling missing from the CSR pattern")
+        pos[ok] = k - indptr[r_i[ok]]
+        item |= pos.astype(np.uint64) << np.uint64(40 + 8 * b)
+    order = np.lexsort((e_i, r_i))
+    inc_ptr = np.zeros(n_free + 1, dtype=np.int64)
+    inc_ptr[1:] = np.cumsum(np.bincount(r_i, minlength=n_free))
+    return PatchPlan(
+        n_patch=0,
+        rows_per_patch=0,
+        chunk=64,
+        prow=np.zeros(1, dtype=np.int32),
+        pelem_ptr=np.zeros(1, dtype=np.int32),
+        pelem=np.zeros(1, dtype=np.int32),
+        elem_order=elem_order,
+        inc_ptr=inc_ptr.astype(np.int32),
+        inc=item[order],
+        inc_aux=None,
+        max_patch_nnz=0,
+        max_row_len=int(row_len.max(initial=0)),
+    )
+
+
+def color_patch_elements(pelem_ptr: np.ndarray, owned_rows: np.ndarray, n_rows: int) -> tuple[np.ndarray, np.ndarray]:
+    """Greedy per-patch coloring (host C++ routine of libfne_hip): colors[i], colors-per-patch."""
+    import ctypes
+
+    from . import native
+
+    n_patch = int(pelem_ptr.size - 1)
+    ptr64 = np.ascontiguousarray(pelem_ptr, dtype=np.int64)
+    rows = np.ascontiguousarray(owned_rows, dtype=np.int64)
+    colors = np.zeros(rows.shape[0], dtype=np.int32)
+    rc = native.lib().fne_color_patch_elements(
+        n_patch, ptr64.ctypes.data_as(ctypes.c_void_p), rows.ctypes.data_as(ctypes.c_void_p),
+        int(rows.shape[1]), int(n_rows), colors.ctypes.data_as(ctypes.c_void_p),
+    )
+    if rc < 0:
+        native.check(rc, "fne_color_patch_elements")
+    pid = np.repeat(np.arange(n_patch), np.diff(ptr64))
+    ncol = np.zeros(n_patch, dtype=np.int64)
+    if colors.size:
+        np.maximum.at(ncol, pid, colors.astype(np.int64) + 1)
+    return colors, ncol
+
+
+def build_p1_colored_plan(
+    elem_free: np.ndarray,
+    indptr: np.ndarray,
+    indices: np.ndarray,
+    n_free: int,
+    *,
+    rows_per_patch: int = 192,
+    elems_per_thread: int = 2,
+    block: int = 256,
+) -> PatchPlan:
+    """Colored patch plan for P1 scalar elements (see p1_color_kernel).
+
+    Item per patch element (uint64): pos_ab at bits 5(3a+b) (column position of vertex b inside
+    the row of vertex a when that row belongs to the patch, else 31) | energy owner at bit 48 |
+    color at bits 52..59.  ``chunk`` of the returned plan holds the elements per thread.
+    """
+    elem_free = np.asarray(elem_free, dtype=np.int64)
+    n_elem, n_loc = elem_free.shape
+    if n_loc != 3:
+        raise ValueError("build_p1_colored_plan expects 3 DOFs per element")
+    if rows_per_patch > block:
+        raise ValueError("rows_per_patch must not exceed the workgroup size")
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    row_len = np.diff(indptr)
+    if int(row_len.max(initial=0)) >= 31:
+        raise ValueError("CSR rows longer than 30 entries do not fit the 5-bit positions")
+    prow = _patch_rows(n_free, rows_per_patch)
+    n_patch = prow.size - 1
+    elem_order, _, ef, pelem, pelem_ptr, _, _, pown = _elements_by_patch(elem_free, rows_per_patch, n_patch)
+    if np.diff(pelem_ptr).max(initial=0) > elems_per_thread * block:
+        raise ValueError("patch has more elements than elems_per_thread * block")
+    pid = np.repeat(np.arange(n_patch), np.diff(pelem_ptr))
+    efp = ef[pelem]  # (n_pe, 3) rows of each patch element
+    lo = prow[pid][:, None]
+    hi = prow[pid + 1][:, None]
+    owned = (efp >= lo) & (efp < hi)
+    owned_rows = np.where(owned, efp, -1)
+    colors, ncol = color_patch_elements(pelem_ptr, owned_rows, n_free)
+    # order each patch's elements by color: the kernel's color steps then touch few waves
+    # (a wave whose 64 slots mix many colors would execute the scatter once per color)
+    slot0 = np.arange(pelem.size) - pelem_ptr[pid]
+    owner_bit = slot0 < pown[pid]
+    reorder = np.lexsort((pelem, colors, pid))
+    pelem, efp, owned, colors, owner_bit = pelem[reorder], efp[reorder], owned[reorder], colors[reorder], owner_bit[reorder]
+    csr_keys = np.repeat(np.arange(n_free, dtype=np.int64), row_len) * np.int64(n_free) + indices
+    item = np.zeros(pelem.size, dtype=np.uint64)
+    for a in range(3):
+        ra = efp[:, a]
+        for b in range(3):
+            cb = efp[:, b]
+            pos = np.full(pelem.size, 31, dtype=np.int64)
+            ok = owned[:, a] & (cb >= 0)
+            key = ra[ok] * np.int64(n_free) + cb[ok]
+            k = np.searchsorted(csr_keys, key)
+            if not np.array_equal(csr_keys[k], key):
+                raise RuntimeError("element coupling missing from the CSR pattern")
+            pos[ok] = k - indptr[ra[ok]]
+            item |= pos.astype(np.uint64) << np.uint64(5 * (3 * a + b))
+    item |= owner_bit.astype(np.uint64) << np.uint64(48)
+    item |= colors.astype(np.uint64) << np.uint64(52)
+    patch_nnz = indptr[prow[1:]] - indptr[prow[:-1]]
+    plan = PatchPlan(
+        n_patch=n_patch,
+        rows_per_patch=rows_per_patch,
+        chunk=int(elems_per_thread),
+        prow=prow.astype(np.int32),
+        pelem_ptr=pelem_ptr.astype(np.int32),
+        pelem=pelem.astype(np.int32),
+        elem_order=elem_order,
+        inc_ptr=np.zeros(1, dtype=np.int32),
+        inc=np.zeros(1, dtype=np.uint32),
+        inc_aux=None,
+        max_patch_nnz=int(patch_nnz.max(initial=0)),
+        max_row_len=int(row_len.max(initial=0)),
+        pown=pown.astype(np.int32),
+    )
+    plan.pitem = item
+    plan.pcolors = ncol.astype(np.int32)
+    return plan

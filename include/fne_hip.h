@@ -79,6 +79,14 @@ typedef struct fne_plan_desc {
   const int32_t* inc_ptr;    /* [n_inc_rows+1] incidence list offsets, one list per assembly row */
   const uint32_t* inc;       /* incidence items (packed slot/local index/column positions) */
   const int32_t* inc_aux;    /* family-specific auxiliary table (NULL for P1) */
+  /* patch-major copies of the element data (owned elements of a patch first, then its halo),
+     so a patch workgroup streams its elements without an indirection */
+  int64_t n_pe;              /* total patch elements = pelem_ptr[n_patch] */
+  const int32_t* pedof;      /* [n_loc][n_pe] */
+  const double* pgeo;        /* [n_geo][n_pe] */
+  const int32_t* pown;       /* [n_patch] owned elements per patch (energy is summed over these) */
+  const uint64_t* pitem;     /* [n_pe] colored-patch item per patch element (FNE_FLAG_COLORED) */
+  const int32_t* pcolors;    /* [n_patch] number of colors of each patch (FNE_FLAG_COLORED) */
   /* problem parameters */
   double param[8];        /* p | eps | C1,D1 | E,nu,phi_deg,cohesion,reg */
   const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]) */
@@ -87,6 +95,13 @@ typedef struct fne_plan_desc {
 } fne_plan_desc;
 
 #define FNE_FLAG_ABS_DET 1 /* Neo-Hookean: |det F| (serial path hyperelasticity/jax/jax_energy.py:26) */
+#define FNE_FLAG_COLORED 4 /* P1: colored patch assembly.  pitem = pos_ab[5(3a+b) : +5) (31 = none)
+                               | owned-row mask [45:48) | energy owner [48] | color [52:60) */
+#define FNE_FLAG_STAGED 8 /* P1: staged patch assembly.  Elements of a patch are evaluated into LDS,
+                              then one thread per row sums its incidence list (inc_ptr/inc: slot[0:13)
+                              | a[13:15) | pos_b[15+5b:20+5b), 31 = none) in registers */
+#define FNE_FLAG_ROW_GATHER 2 /* P1: one thread per CSR row gathers its incident elements (no LDS
+                                 staging); inc holds uint64 items e[0:32) a[32:34) pos_b[40+8b:48+8b) */
 
 typedef struct fne_plan fne_plan;
 
@@ -127,6 +142,15 @@ int fne_hessian(fne_plan* plan, const double* x, const double* dvals, const doub
                 double* g, void* vals, int32_t precision, fne_stream_t stream);
 
 /*
+ * Fused Newton-step assembly: energy (out, FP64 scalar, may be NULL), gradient (g, may be
+ * NULL) and Hessian values (vals) in one pass over the elements.  Same definitions as
+ * fne_energy / fne_gradient / fne_hessian.
+ */
+int fne_energy_grad_hessian(fne_plan* plan, const double* x, const double* dvals, const double* f,
+                            double* out, double* g, void* vals, int32_t precision,
+                            fne_stream_t stream);
+
+/*
  * Per-element energy / gradient / Hessian (dense, for the autograd boundary and tests):
  * e_out[n_elem] (may be NULL), ge_out[n_loc][n_elem] (may be NULL),
  * he_out[n_loc*n_loc][n_elem] (may be NULL).  Replaces the _make_local_element_kernels
@@ -148,6 +172,17 @@ int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, 
 
 /* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
 int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
+
+/*
+ * Host (CPU) helper for plan construction: greedy coloring of patch elements so that no two
+ * elements of one color share an owned row of their patch.  rows[3*i+a] is the owned row of
+ * local vertex a of patch element i (-1 if not owned); patch elements are grouped by patch via
+ * pelem_ptr.  colors[i] receives the color (0..63).  Returns the largest color count of a patch,
+ * or FNE_EINVAL if some patch needs more than 64 colors.  Replaces nothing in the reference
+ * (its scatter is serial np.add.at); the coloring is what makes the GPU reduction race-free.
+ */
+int fne_color_patch_elements(int32_t n_patch, const int64_t* pelem_ptr, const int64_t* rows,
+                             int32_t n_loc, int64_t n_rows, int32_t* colors);
 
 const char* fne_last_error(void);
 /* Library/ABI version string. */

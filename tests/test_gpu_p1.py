@@ -34,18 +34,22 @@ def _check_all(asm, kind, d, u_orig, rtol_e=1e-13, rtol=1e-12):
     return H
 
 
-@pytest.mark.parametrize("mode", ["block_morton", "block_xyz"])
-def test_plaplace_reference_mesh_parity(require_gpu, mode):
+@pytest.mark.parametrize(
+    "mode,amode",
+    [("block_morton", "patch"), ("block_xyz", "patch"), ("block_morton", "staged"), ("block_morton", "rowgather")],
+)
+def test_plaplace_reference_mesh_parity(require_gpu, mode, amode):
     d, adj = fixtures.plaplace(5)
-    asm = PLaplaceReorderedElementAssembler(d, None, adj, reorder_mode=mode)
+    asm = PLaplaceReorderedElementAssembler(d, None, adj, reorder_mode=mode, assembly_mode=amode)
     u = np.random.default_rng(11).random(d["freedofs"].size)
     _check_all(asm, "plaplace", d, u)
     asm.cleanup()
 
 
-def test_gl_reference_mesh_parity(require_gpu):
+@pytest.mark.parametrize("amode", ["patch", "staged", "rowgather"])
+def test_gl_reference_mesh_parity(require_gpu, amode):
     d, adj = fixtures.gl(5)
-    asm = GLReorderedElementAssembler(d, None, adj)
+    asm = GLReorderedElementAssembler(d, None, adj, assembly_mode=amode)
     u = np.random.default_rng(12).uniform(-1, 1, d["freedofs"].size)
     _check_all(asm, "gl", d, u)
     asm.cleanup()
@@ -104,3 +108,24 @@ def test_spmv_matches_scipy(require_gpu):
     asm.A.mult(asm.create_vec(xin), y)
     ref = A @ xin
     assert np.allclose(y.array, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("amode", ["patch", "staged", "rowgather"])
+def test_fused_energy_grad_hessian_matches_separate_calls(require_gpu, amode):
+    """fne_energy_grad_hessian == fne_energy + fne_gradient + fne_hessian (GL has only-constrained
+    corner elements, whose energy the fused pass must still count)."""
+    gp, gadj = build_gl_problem(24)
+    asm = GLReorderedElementAssembler(gp, None, gadj, hessian_precision="fp32", assembly_mode=amode)
+    u = np.random.default_rng(8).uniform(-1, 1, gp["freedofs"].size)
+    x = asm.create_vec(u[asm.layout.perm])
+    g1, g2 = x.duplicate(), x.duplicate()
+    e_sep = float(asm.energy_device(x.t).item())
+    asm.gradient_device(x.t, g1.t)
+    asm.hessian_device(x.t)
+    h_sep = asm.A.data.clone()
+    e_fused = float(asm.energy_grad_hessian_device(x.t, g2.t).item())
+    assert abs(e_fused - e_sep) <= 1e-14 * abs(e_sep)
+    assert torch.equal(g1.t, g2.t)
+    assert torch.equal(h_sep, asm.A.data)
+    ref = assembly.assemble("gl", gp, u)
+    assert abs(e_fused - ref["e"]) <= 1e-13 * abs(ref["e"])

