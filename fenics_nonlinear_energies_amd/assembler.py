@@ -258,7 +258,8 @@ class ReorderedElementAssemblerBase:
         if plan.n_patch == 0 or (self._single_pass_staging() and self.assembly_mode == "patch"):
             budget = 1 << 30  # no LDS element staging
         per_elem = self._lds_bytes_per_element()
-        slice_bytes = 0 if self.assembly_mode == "staged" else self._lds_slice_bytes_per_entry() * int(plan.max_patch_nnz)
+        p1_staged = self.assembly_mode == "staged" and self.block_size == 1
+        slice_bytes = 0 if p1_staged else self._lds_slice_bytes_per_entry() * int(plan.max_patch_nnz)
         avail = budget - slice_bytes - 64
         if plan.chunk * per_elem > avail:
             if self._single_pass_staging():

@@ -223,9 +223,12 @@ def test_host_plans_fit_kernel_limits(prec):
     for cls, p, adj in ((GLReorderedElementAssembler, gp, gadj), (HEReorderedElementAssembler, hp, hadj)):
         asm = cls(p, None, adj, hessian_precision=prec, device="host-plan")
         plan = asm.patch_plan
-        if asm.block_size == 1:  # colored P1 kernel: elements held in registers, chunk = per thread
-            assert 1 <= plan.chunk <= 3 and plan.max_patch_elems <= 256 * plan.chunk
-            assert 8 * plan.max_patch_nnz + 12 * 257 <= asm.lds_budget_bytes()
+        if asm.block_size == 1:  # staged P1 kernel: every patch element staged in LDS at once
+            hsz = 4 if prec == "fp32" else 8
+            assert plan.chunk >= plan.max_patch_elems and plan.chunk % 64 == 0
+            lds = 24 * plan.chunk + hsz * (((plan.max_patch_nnz + 1) & ~1) + 6 * plan.chunk)
+            assert lds <= 160 * 1024  # gfx950 workgroup LDS limit
+            assert int(np.max(np.diff(lay_indptr := asm.layout.indptr))) <= 30
         else:
             lds = asm._lds_slice_bytes_per_entry() * plan.max_patch_nnz + asm._lds_bytes_per_element() * plan.chunk
             assert lds <= asm.lds_budget_bytes() and plan.chunk >= 64
