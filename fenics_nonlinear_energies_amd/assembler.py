@@ -298,9 +298,10 @@ class ReorderedElementAssemblerBase:
             self._dev["pedof"] = self._t(codes[pe].T, np.int32)
             self._dev["pgeo"] = self._t(geo[:, pe], np.float64)
             self._dev["pown"] = self._t(plan.pown, np.int32)
-            if getattr(plan, "pitem", None) is not None:
-                self._dev["pitem"] = self._t(plan.pitem.view(np.int64), np.int64)
-                self._dev["pcolors"] = self._t(plan.pcolors, np.int32)
+        if getattr(plan, "pitem", None) is not None:
+            self._dev["pitem"] = self._t(np.ascontiguousarray(plan.pitem).view(np.int64), np.int64)
+        if getattr(plan, "pcolors", None) is not None:
+            self._dev["pcolors"] = self._t(plan.pcolors, np.int32)
         q = self._quadrature_table()
         if q is not None:
             self._dev["qdata"] = self._t(q, np.float64)

@@ -34,6 +34,10 @@ struct fne_plan {
   int device = 0;
   int mode = 0;  // constitutive mode (Mohr-Coulomb: 0 full, 1 elastic)
   size_t max_lds = 65536;  // dynamic LDS bytes one workgroup may request on this device
+  // Mohr-Coulomb: per-element gradient [n_elem][32] (FP64) and element matrices
+  // [n_elem][n_loc][n_loc] (FP64 or FP32), written by the element kernel, read by the row gather
+  double* mc_g = nullptr;
+  void* mc_h = nullptr;
 };
 
 constexpr int kEnergyBlocks = 2048;  // fixed grid => fixed reduction order => deterministic
@@ -130,5 +134,7 @@ struct FneCall {
 
 int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c);
 int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c);
-int fne_launch_p4(fne_plan* plan, FneOp op, const FneCall& c);
+int fne_launch_mc(fne_plan* plan, FneOp op, const FneCall& c);
 int fne_finalize_energy(fne_plan* plan, int nblocks, double* out, hipStream_t stream);
+// bytes of the Mohr-Coulomb element-matrix scratch (and, via gbytes, of the gradient scratch)
+size_t fne_mc_scratch_bytes(const fne_plan_desc& d, size_t* gbytes);

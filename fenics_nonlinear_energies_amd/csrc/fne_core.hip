@@ -136,8 +136,16 @@ int validate_desc(const fne_plan_desc* d) {
       }
       break;
     case FNE_KIND_MOHR_COULOMB:
-      if (d->n_loc != 30) {
-        fne_set_error("Mohr-Coulomb P4 plans need n_loc=30");
+      if (d->n_loc != 30 && d->n_loc != 12) {
+        fne_set_error("Mohr-Coulomb plans need n_loc=30 (P4) or 12 (P2)");
+        return FNE_EINVAL;
+      }
+      if (d->nq < 1 || d->nq > 21 || !d->qdata) {
+        fne_set_error("Mohr-Coulomb plans need 1..21 quadrature points and a quadrature table");
+        return FNE_EINVAL;
+      }
+      if (d->n_geo != 5 && d->n_geo != 5 + 3 * d->nq) {
+        fne_set_error("Mohr-Coulomb plans need n_geo = 5 (K, |detJ|) or 5 + 3*nq (with eps_p_old)");
         return FNE_EINVAL;
       }
       break;
@@ -166,7 +174,7 @@ int dispatch(fne_plan* plan, FneOp op, const FneCall& c) {
     case FNE_KIND_NEOHOOKEAN:
       return fne_launch_tet(plan, op, c);
     case FNE_KIND_MOHR_COULOMB:
-      return fne_launch_p4(plan, op, c);
+      return fne_launch_mc(plan, op, c);
     default:
       fne_set_error("unknown energy kind");
       return FNE_EUNSUPPORTED;
@@ -224,6 +232,17 @@ int fne_plan_create(const fne_plan_desc* desc, fne_plan** out) {
     fne_set_error("hipMalloc of reduction scratch failed");
     return FNE_ENOMEM;
   }
+  if (desc->kind == FNE_KIND_MOHR_COULOMB && desc->n_elem > 0) {
+    size_t gbytes = 0;
+    const size_t hbytes = fne_mc_scratch_bytes(*desc, &gbytes);
+    if (hipMalloc(&p->mc_h, hbytes) != hipSuccess || hipMalloc(&p->mc_g, gbytes) != hipSuccess) {
+      if (p->mc_h) (void)hipFree(p->mc_h);
+      (void)hipFree(p->partials);
+      delete p;
+      fne_set_error("hipMalloc of the element scratch failed");
+      return FNE_ENOMEM;
+    }
+  }
   *out = p;
   return FNE_OK;
 }
@@ -231,6 +250,8 @@ int fne_plan_create(const fne_plan_desc* desc, fne_plan** out) {
 int fne_plan_destroy(fne_plan* plan) {
   if (!plan) return FNE_OK;
   if (plan->partials) (void)hipFree(plan->partials);
+  if (plan->mc_h) (void)hipFree(plan->mc_h);
+  if (plan->mc_g) (void)hipFree(plan->mc_g);
   delete plan;
   return FNE_OK;
 }

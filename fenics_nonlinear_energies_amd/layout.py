@@ -171,8 +171,11 @@ def csr_from_keys(keys: np.ndarray, n: int) -> tuple[np.ndarray, np.ndarray]:
     return indptr, cols
 
 
-def element_pattern_keys(elem_free: np.ndarray, n_free: int, chunk: int = 1 << 18) -> np.ndarray:
+def element_pattern_keys(elem_free: np.ndarray, n_free: int, chunk: int | None = None) -> np.ndarray:
     """Sorted unique row*n+col keys of all (free, free) pairs of every element."""
+    if chunk is None:  # about 16M candidate pairs per block
+        nl = max(1, int(elem_free.shape[1]) if elem_free.ndim == 2 else 1)
+        chunk = max(1024, (1 << 24) // (nl * nl))
     parts = []
     n = np.int64(n_free)
     for s in range(0, elem_free.shape[0], chunk):

@@ -108,6 +108,21 @@ _SIGNATURES = [
         C.c_int,
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p],
     ),
+    ("fne_mlp_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fne_mlp_workspace_bytes", C.c_int64, [C.c_void_p, C.c_int64]),
+    (
+        "fne_mlp_backward",
+        C.c_int,
+        [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+         C.c_void_p],
+    ),
+    (
+        "fne_mlp_adam",
+        C.c_int,
+        [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float,
+         C.c_float, C.c_float, C.c_int32, C.c_void_p],
+    ),
+    ("fne_mlp_to_bf16", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("fne_last_error", C.c_char_p, []),
     ("fne_version", C.c_char_p, []),
 ]

@@ -423,3 +423,86 @@ def build_p1_colored_plan(
     plan.pitem = item
     plan.pcolors = ncol.astype(np.int32)
     return plan
+
+
+ROW_POS_NONE = 255
+
+
+@dataclass
+class RowIncidencePlan(PatchPlan):
+    """Per-row incidence lists for elements with up to 32 DOFs (row-gather assembly).
+
+    ``inc[k]`` = element * 32 + local DOF of the k-th (row, element) incidence, rows in
+    CSR order and elements ascending within a row; ``pitem`` holds, per incidence, the 32
+    byte positions of the element's local DOFs inside the row (255 = constrained / none).
+    """
+
+    pitem: np.ndarray | None = None  # uint8 [n_items, 32]
+
+
+def build_row_incidence_plan(elem_free: np.ndarray, indptr: np.ndarray, indices: np.ndarray, n_free: int,
+                             *, item_chunk: int = 1 << 20) -> RowIncidencePlan:
+    """Row-gather schedule: each CSR row sums the rows of its incident element matrices.
+
+    Replaces the reference's scatter plan (``_build_scatter_data``,
+    core/petsc/reordered_element_base.py:1464-1534): the same (element, i, j) -> CSR
+    position map, grouped by destination row so that the device reduction is a fixed-order
+    gather (no atomics).  Elements are renumbered by their smallest free row for locality.
+    """
+    elem_free = np.asarray(elem_free, dtype=np.int64)
+    n_elem, n_loc = elem_free.shape
+    if n_loc > 32:
+        raise ValueError("row incidence plan supports at most 32 DOFs per element")
+    if n_elem >= (1 << 27):
+        raise ValueError("too many elements for 27-bit element ids")
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    row_len = np.diff(indptr)
+    max_row_len = int(row_len.max(initial=0))
+    if max_row_len >= ROW_POS_NONE:
+        raise ValueError("CSR rows longer than 254 entries are not supported by 8-bit positions")
+    masked = np.where(elem_free >= 0, elem_free, n_free)
+    key = masked.min(axis=1) if n_elem else np.zeros(0, dtype=np.int64)
+    elem_order = np.argsort(key, kind="stable").astype(np.int64)
+    ef = elem_free[elem_order]
+    e_i, c_i = np.nonzero(ef >= 0)
+    r_i = ef[e_i, c_i]
+    order = np.lexsort((c_i, e_i, r_i))
+    e_i, c_i, r_i = e_i[order], c_i[order], r_i[order]
+    inc_ptr = np.zeros(n_free + 1, dtype=np.int64)
+    inc_ptr[1:] = np.cumsum(np.bincount(r_i, minlength=n_free))
+    inc = (e_i.astype(np.uint64) * np.uint64(32) + c_i.astype(np.uint64)).astype(np.uint32)
+    n_items = e_i.size
+    pos = np.full((n_items, 32), ROW_POS_NONE, dtype=np.uint8)
+    nn = np.int64(n_free)
+    csr_keys = np.repeat(np.arange(n_free, dtype=np.int64), row_len) * nn + indices  # sorted
+    for s in range(0, n_items, item_chunk):
+        sl = slice(s, min(n_items, s + item_chunk))
+        cols = ef[e_i[sl]]  # (k, n_loc)
+        rows = r_i[sl]
+        valid = cols >= 0
+        k = np.searchsorted(csr_keys, (rows[:, None] * nn + np.where(valid, cols, 0)).ravel()).reshape(cols.shape)
+        k = np.minimum(k, max(csr_keys.size - 1, 0))
+        rel = k - indptr[rows][:, None]
+        ok = valid & (csr_keys[k] == rows[:, None] * nn + cols)
+        if np.any(valid & ~ok):
+            raise ValueError("element coupling missing from the CSR pattern")
+        pos[sl, :n_loc] = np.where(ok, rel, ROW_POS_NONE).astype(np.uint8)
+    empty = np.zeros(1, dtype=np.int32)
+    return RowIncidencePlan(
+        n_patch=0,
+        rows_per_patch=0,
+        chunk=0,
+        prow=empty,
+        pelem_ptr=empty,
+        pelem=np.zeros(0, dtype=np.int32),
+        elem_order=elem_order,
+        inc_ptr=inc_ptr.astype(np.int32),
+        inc=inc,
+        inc_aux=None,
+        max_patch_nnz=0,
+        max_row_len=max_row_len,
+        pown=None,
+        pitem=pos,
+    )
+

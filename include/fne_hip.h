@@ -184,6 +184,41 @@ int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream
 int fne_color_patch_elements(int32_t n_patch, const int64_t* pelem_ptr, const int64_t* rows,
                              int32_t n_loc, int64_t n_rows, int32_t* colors);
 
+/*
+ * Learned warm start: per-node MLP (BF16 MFMA products, FP32 activations and optimiser state).
+ * No counterpart in the reference, which warm-starts every load / continuation step from the
+ * previous converged state (core/petsc/load_step_driver.py:58-110,
+ * hyperelasticity/jax_petsc/solver.py:351-359); that state stays the fallback.
+ * Layer l maps dims[l] -> dims[l+1]: y = W_l x + b_l, W_l [dims[l+1]][dims[l]] BF16 bits,
+ * b_l FP32, tanh after every layer but the last.  Row-major node arrays: x [n][dims[0]].
+ */
+#define FNE_MLP_MAX_LAYERS 8
+typedef struct fne_mlp_desc {
+  int32_t n_layers;
+  int32_t dims[FNE_MLP_MAX_LAYERS + 1];
+  const uint16_t* w[FNE_MLP_MAX_LAYERS];
+  const float* b[FNE_MLP_MAX_LAYERS];
+} fne_mlp_desc;
+
+/* y [n][dims[L]]; acts (optional unless training) receives the hidden activations of layers
+ * 0..L-2 back to back ([n][dims[1]], [n][dims[2]], ...). */
+int fne_mlp_forward(const fne_mlp_desc* net, int64_t n, const float* x, float* y, float* acts,
+                    fne_stream_t stream);
+/* Bytes of the workspace fne_mlp_backward needs for n nodes (negative on bad arguments). */
+int64_t fne_mlp_workspace_bytes(const fne_mlp_desc* net, int64_t n);
+/* Gradients of sum_i dy[i] . y[i]: dw[l] [dims[l+1]][dims[l]] and db[l] [dims[l+1]] (FP32),
+ * from the inputs x and the activations saved by fne_mlp_forward.  Deterministic. */
+int fne_mlp_backward(const fne_mlp_desc* net, int64_t n, const float* x, const float* acts,
+                     const float* dy, float* const* dw, float* const* db, void* workspace,
+                     int64_t ws_bytes, fne_stream_t stream);
+/* One Adam step (bias-corrected; weight_decay adds weight_decay * w to the gradient) on n FP32
+ * master weights; w_bf16 (may be NULL) receives the rounded copy used by the products. */
+int fne_mlp_adam(int64_t n, float* w, uint16_t* w_bf16, const float* grad, float* m, float* v,
+                 float lr, float beta1, float beta2, float eps, float weight_decay, int32_t step,
+                 fne_stream_t stream);
+/* Round FP32 weights to BF16 (nearest even). */
+int fne_mlp_to_bf16(int64_t n, const float* w, uint16_t* w_bf16, fne_stream_t stream);
+
 const char* fne_last_error(void);
 /* Library/ABI version string. */
 const char* fne_version(void);

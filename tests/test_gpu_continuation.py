@@ -1,0 +1,57 @@
+"""Mohr-Coulomb strength-reduction continuation on the GPU against the CPU oracle.
+
+The elastic initial guess is checked against a direct solve of the oracle's elastic
+stiffness (rel. 1e-7, CG tolerance 1e-10).  Each continuation step is converged tightly
+(gradient norm 1e-9 with KSP rtol 1e-10) and its final energy is compared with the oracle's
+golden-section Newton (direct solves) from the same initial state: rel. 1e-10 (the
+perfectly plastic energy is convex, so both reach the same minimum).
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from fenics_nonlinear_energies_amd.continuation import elastic_initial_guess, run_lambda_continuation
+from fenics_nonlinear_energies_amd.problems.slope_stability import (
+    SlopeStabilityReorderedElementAssembler,
+    davis_b_reduction,
+)
+from oracle import assembly, solvers
+from tests.test_oracle_mc import mc_params
+
+pytestmark = pytest.mark.gpu
+
+
+def test_elastic_guess_and_continuation_match_oracle(require_gpu):
+    p, adj = mc_params(h=4.0, degree=2, lam=1.0)
+    asm = SlopeStabilityReorderedElementAssembler(p, None, adj, hessian_precision="fp64", ksp_type="cg")
+    lay = asm.layout
+    x, its = elastic_initial_guess(asm, rtol=1e-12, max_it=50000)
+    K = assembly.assemble("mc", p, np.zeros(lay.n_free), want=("h",), elastic_only=True)["h"]
+    f = p["force"][p["freedofs"]]
+    u_el = spla.spsolve(K.tocsc(), f)
+    assert np.abs(x.array[lay.iperm] - u_el).max() <= 1e-7 * np.abs(u_el).max()
+
+    lambdas = [1.0, 1.05, 1.1]  # below the factor of safety (about 1.2 for this slope)
+    recs = run_lambda_continuation(
+        asm, lambdas, x, ksp_rtol=1e-10, ksp_max_it=50000,
+        newton_kwargs=dict(tolg=1e-9, tolg_rel=0.0, tolf=0.0, tolx_rel=0.0, tolx_abs=0.0, maxit=60,
+                           require_all_convergence=False),
+    )
+    assert all(r["message"] == "Gradient norm converged" for r in recs), recs
+    u_prev = u_el
+    for lam, rec in zip(lambdas, recs):
+        q = dict(p)
+        q["cohesion"], q["phi_deg"] = davis_b_reduction(p["c0"], p["phi_deg_raw"], p["psi_deg"], lam)
+        E = lambda u: assembly.assemble("mc", q, u, want=("e",))["e"]
+        G = lambda u: assembly.assemble("mc", q, u, want=("g",))["g"]
+
+        def solve(u, rhs):
+            H = assembly.assemble("mc", q, u, want=("h",))["h"]
+            return spla.spsolve(H.tocsc(), rhs), 1
+
+        res = solvers.newton_golden(E, G, solve, u_prev, tolf=0.0, tolg=1e-9, tolx_rel=0.0, tolx_abs=0.0,
+                                    maxit=60, require_all=False)
+        assert abs(rec["energy"] - res["fun"]) <= 1e-10 * abs(res["fun"]), (rec, res["fun"])
+        u_prev = res["x"]
+    asm.cleanup()
