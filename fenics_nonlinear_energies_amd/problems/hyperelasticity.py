@@ -184,3 +184,62 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
         nodes = int(max(1, min(85, nodes if rows_per_patch is None else int(rows_per_patch) // 3)))
         return build_block_plan(elem_free, lay.indptr, lay.indices, lay.n_free, block_size=3,
                                 nodes_per_patch=nodes, chunk=int(chunk or 128))
+
+
+HE_NEWTON = dict(tolf=1e-4, tolg=1e-3, tolg_rel=1e-3, maxit=100, linesearch_interval=(-0.5, 2.0),
+                 linesearch_tol=1e-3)
+
+
+def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * np.pi, *, num_steps=None,
+                        predictor=None, on_step=None, ksp_rtol: float = 1e-1, ksp_max_it: int = 30,
+                        newton_kwargs: dict | None = None):
+    """Rotate the right face in ``total_steps`` equal steps and solve each step with Newton.
+
+    Follows the reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-359``):
+    angle_k = k * total_angle / total_steps, the rotated face is imposed through
+    ``update_dirichlet`` and the step starts from the previous converged state.
+    ``predictor(history, angle)`` may return another initial guess (device tensor of free
+    values); ``history`` holds (angle, free-state tensor) of the converged steps, starting with
+    the reference configuration at angle 0.  The reference rotates by 4 * 2 pi in total
+    (``solver.py:225``); the angle is a parameter here.
+    """
+    import time
+
+    import torch
+
+    from ..drivers import solve_newton
+
+    p = asm.params
+    opts = dict(HE_NEWTON)
+    opts.update(newton_kwargs or {})
+    lay = asm.layout
+    ref = np.asarray(p["u_0_ref"], dtype=np.float64)
+    x = asm.create_vec(ref[p["freedofs"]][lay.perm])
+    history = [(0.0, x.t.clone())]
+    records = []
+    n_run = int(total_steps if num_steps is None else num_steps)
+    for k in range(1, n_run + 1):
+        angle = k * float(total_angle) / float(total_steps)
+        asm.update_dirichlet(rotate_right_face_from_reference(ref, p["nodes2coord"], angle, p["right_nodes"]))
+        guess = predictor(history, angle) if predictor is not None else None
+        if guess is not None:
+            x.t.copy_(guess)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, **opts)
+        torch.cuda.synchronize()
+        rec = {
+            "step": k,
+            "angle": angle,
+            "nit": int(res.get("nit", 0)),
+            "message": str(res.get("message", "")),
+            "energy": float(res.get("fun", np.nan)),
+            "linear_iters": int(res.get("linear_iters", 0)),
+            "time": time.perf_counter() - t0,
+            "predictor": guess is not None,
+        }
+        records.append(rec)
+        history.append((angle, x.t.clone()))
+        if on_step is not None:
+            on_step(rec, history)
+    return records, x

@@ -159,9 +159,76 @@ class NodeMLP:
         return losses
 
 
-def he_features(X: np.ndarray, u_k: np.ndarray, u_km1: np.ndarray, load_next: float, scale: float) -> np.ndarray:
-    """Per-node inputs (n, 10): coordinates, last two displacements (all / scale) and the next
-    load parameter (e.g. rotation angle / total angle).  Displacements are (n, 3)."""
-    n = X.shape[0]
-    lp = np.full((n, 1), float(load_next))
-    return np.concatenate((X / scale, u_k / scale, u_km1 / scale, lp), axis=1).astype(np.float32)
+class NodalPredictor:
+    """Initial guess for the next load / continuation step from the converged history.
+
+    ``mode``: "previous" (the reference's choice: last converged state), "extrapolate"
+    (u_k + (u_k - u_{k-1}) for equally spaced steps), "learned" (extrapolation plus the MLP
+    correction).  Works on the free DOFs of the assembler's reordered numbering; per-node
+    features use the nodes whose DOFs are all free (``free_node_dofs``).
+
+    ``state_offset`` (free values) is subtracted to form displacements (the HE unknown is the
+    deformed position, so the offset is the reference position; zero for slope displacements).
+    """
+
+    def __init__(self, free_node_dofs: np.ndarray, coords: np.ndarray, *, state_offset, disp_scale: float,
+                 load_scale: float, net: NodeMLP | None = None, mode: str = "learned"):
+        self.idx = torch.as_tensor(np.asarray(free_node_dofs, dtype=np.int64))  # (n_nodes_free, dim)
+        self.coords = np.asarray(coords, dtype=np.float64)
+        self.dim = self.idx.shape[1]
+        self.offset = state_offset
+        self.disp_scale = float(disp_scale)
+        self.load_scale = float(load_scale)
+        self.net = net
+        self.mode = str(mode)
+        self._dev = None
+
+    def _to(self, t):
+        if self._dev is None:
+            self._dev = self.idx.to(t.device)
+            self._X = torch.as_tensor(self.coords, dtype=torch.float32, device=t.device)
+        return self._dev
+
+    def features(self, u_k: torch.Tensor, u_km1: torch.Tensor, load_next: float) -> torch.Tensor:
+        idx = self._to(u_k)
+        dk = (u_k - self.offset)[idx].float() / self.disp_scale
+        dkm1 = (u_km1 - self.offset)[idx].float() / self.disp_scale
+        lp = torch.full((idx.shape[0], 1), float(load_next) / self.load_scale, device=u_k.device)
+        return torch.cat((self._X / self.coord_scale, dk, dkm1, lp), dim=1).contiguous()
+
+    @property
+    def coord_scale(self) -> float:
+        return float(np.abs(self.coords).max()) or 1.0
+
+    @property
+    def n_in(self) -> int:
+        return 3 * self.dim + 1
+
+    def target(self, u_k, u_km1, u_next) -> torch.Tensor:
+        """Training target: (u_{k+1} - extrapolation) per free node, scaled."""
+        idx = self._to(u_k)
+        pred = 2.0 * u_k - u_km1
+        return ((u_next - pred)[idx].float() / self.disp_scale).contiguous()
+
+    def __call__(self, history, load_next):
+        if self.mode == "previous" or len(history) < 2:
+            return None  # caller keeps the previous converged state
+        u_k, u_km1 = history[-1][1], history[-2][1]
+        guess = 2.0 * u_k - u_km1
+        if self.mode == "extrapolate" or self.net is None:
+            return guess
+        idx = self._to(u_k)
+        corr = self.net.forward(self.features(u_k, u_km1, load_next))
+        guess = guess.clone()
+        guess[idx] += self.disp_scale * corr.double()
+        return guess
+
+
+def training_pairs(predictor: NodalPredictor, history) -> tuple[torch.Tensor, torch.Tensor]:
+    """(features, targets) of every (k-1, k, k+1) triple of a converged history."""
+    xs, ts = [], []
+    for j in range(1, len(history) - 1):
+        (_, u_km1), (_, u_k), (load_next, u_next) = history[j - 1], history[j], history[j + 1]
+        xs.append(predictor.features(u_k, u_km1, load_next))
+        ts.append(predictor.target(u_k, u_km1, u_next))
+    return torch.cat(xs), torch.cat(ts)
