@@ -1,0 +1,331 @@
+// Device-resident preconditioned CG (KSPCG semantics) around the CSR SpMV.
+//
+// PETSc's KSPCG as the reference drives it (scalar_problem_driver.py:154-330): zero initial
+// guess, point-Jacobi left preconditioning (zero diagonal entries -> 1), convergence on the
+// preconditioned residual norm ||z_k|| <= max(rtol ||z_0||, atol), divergence at
+// dtol ||z_0||, indefinite-matrix / indefinite-PC breakdown checks, iteration cap.  Every
+// scalar (alpha, beta, norms, the stop flag) lives in device memory: an iteration is four
+// launches (fused p-update + SpMV + p.w partials; alpha; x/r/z update + z.z and z.r partials;
+// beta and the convergence test) and the host only reads the stop flag every
+// ``check_every`` iterations.  Iterations enqueued after the stop are no-ops, so the result
+// and the iteration count are exactly those of the iteration that met the test.  All dot
+// products are compensated FP64 sums in a fixed order: the solve is bitwise reproducible.
+#include "fne_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+// state slots (doubles)
+enum : int {
+  kBeta = 0,
+  kBetaOld,
+  kDpiOld,
+  kRnorm0,
+  kRnorm,
+  kAlpha,
+  kRatio,
+  kDone,
+  kIts,
+  kReason,
+  kStateLen = 16
+};
+
+constexpr int kConvRtol = 2, kConvAtol = 3, kDivDtol = -4, kDivIndefPc = -8, kDivNan = -9, kDivIndefMat = -10;
+
+struct CgParams {
+  int n;
+  const int32_t* indptr;
+  const int32_t* indices;
+  const double* inv_diag;  // null -> identity
+  double rtol, atol, dtol;
+};
+
+__device__ __forceinline__ int cg_converged(int it, double rn, double rn0, const CgParams& P) {
+  if (!isfinite(rn)) return kDivNan;
+  if (rn <= fmax(P.rtol * rn0, P.atol)) return rn > P.atol ? kConvRtol : kConvAtol;
+  if (it > 0 && rn >= P.dtol * rn0) return kDivDtol;
+  return 0;
+}
+
+__device__ __forceinline__ void merge_partials(const double2* part, int nb, double& s, double& c) {
+  s = 0.0;
+  c = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc_merge(s, c, part[i].x, part[i].y);
+  block_reduce_pair<1024>(s, c);
+}
+
+// r = b, z = D^-1 r, x = 0, p_old = 0; partials of z.z and z.r
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(CgParams P, const double* __restrict__ b,
+                                                         double* __restrict__ x, double* __restrict__ r,
+                                                         double* __restrict__ z, double* __restrict__ p0,
+                                                         double2* __restrict__ part) {
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += stride) {
+    const double ri = b[i];
+    const double zi = P.inv_diag ? ri * P.inv_diag[i] : ri;
+    r[i] = ri;
+    z[i] = zi;
+    x[i] = 0.0;
+    p0[i] = 0.0;
+    acc_add(s1, c1, zi * zi);
+    acc_add(s2, c2, zi * ri);
+  }
+  block_reduce_pair(s1, c1);
+  __syncthreads();
+  block_reduce_pair(s2, c2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = make_double2(s1, c1);
+    part[gridDim.x + blockIdx.x] = make_double2(s2, c2);
+  }
+}
+
+__global__ __launch_bounds__(1024) void cg_init_finalize(CgParams P, const double2* __restrict__ part, int nb,
+                                                         double* __restrict__ st) {
+  double zz, czz, zr, czr;
+  merge_partials(part, nb, zz, czz);
+  __syncthreads();
+  merge_partials(part + nb, nb, zr, czr);
+  if (threadIdx.x != 0) return;
+  const double dp = sqrt(fmax(zz + czz, 0.0));
+  st[kRnorm0] = dp;
+  st[kRnorm] = dp;
+  st[kIts] = 0.0;
+  st[kDpiOld] = NAN;
+  st[kRatio] = 0.0;
+  st[kDone] = 0.0;
+  st[kReason] = 0.0;
+  int reason = cg_converged(0, dp, dp, P);
+  const double beta = zr + czr;
+  if (!reason && beta < 0.0) reason = kDivIndefPc;
+  st[kBeta] = beta;
+  st[kBetaOld] = beta;
+  if (reason) {
+    st[kReason] = reason;
+    st[kDone] = 1.0;
+  }
+}
+
+// p_new = z + ratio p_old; w = A p_new (on the fly from z and p_old); partials of p_new.w
+template <typename VT, int LPR>
+__global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* __restrict__ vals,
+                                                         const double* __restrict__ z,
+                                                         const double* __restrict__ p_old,
+                                                         double* __restrict__ p_new, double* __restrict__ w,
+                                                         const double* __restrict__ st,
+                                                         double2* __restrict__ part) {
+  if (st[kDone] != 0.0) return;
+  const double ratio = st[kRatio];
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = gtid / LPR;
+  const int lane = gtid % LPR;
+  double acc = 0.0;
+  if (row < P.n) {
+    const int kb = P.indptr[row], ke = P.indptr[row + 1];
+    for (int k = kb + lane; k < ke; k += LPR) {
+      const int c = P.indices[k];
+      acc += double(vals[k]) * (z[c] + ratio * p_old[c]);
+    }
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  double s = 0.0, cc = 0.0;
+  if (row < P.n && lane == 0) {
+    const double pn = z[row] + ratio * p_old[row];
+    p_new[row] = pn;
+    w[row] = acc;
+    acc_add(s, cc, pn * acc);
+  }
+  block_reduce_pair(s, cc);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(s, cc);
+}
+
+__global__ __launch_bounds__(1024) void cg_alpha_finalize(const double2* __restrict__ part, int nb,
+                                                          double* __restrict__ st) {
+  if (st[kDone] != 0.0) return;
+  double s, c;
+  merge_partials(part, nb, s, c);
+  if (threadIdx.x != 0) return;
+  const double dpi = s + c;
+  const double old = st[kDpiOld];
+  if (dpi == 0.0 || (isfinite(old) && dpi * old <= 0.0) || !isfinite(dpi)) {
+    st[kReason] = isfinite(dpi) ? kDivIndefMat : kDivNan;
+    st[kDone] = 1.0;
+    return;
+  }
+  st[kDpiOld] = dpi;
+  st[kAlpha] = st[kBeta] / dpi;
+}
+
+// x += alpha p, r -= alpha w, z = D^-1 r; partials of z.z and z.r
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(CgParams P, const double* __restrict__ p,
+                                                           const double* __restrict__ w, double* __restrict__ x,
+                                                           double* __restrict__ r, double* __restrict__ z,
+                                                           const double* __restrict__ st,
+                                                           double2* __restrict__ part) {
+  if (st[kDone] != 0.0) return;
+  const double a = st[kAlpha];
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += stride) {
+    x[i] += a * p[i];
+    const double ri = r[i] - a * w[i];
+    r[i] = ri;
+    const double zi = P.inv_diag ? ri * P.inv_diag[i] : ri;
+    z[i] = zi;
+    acc_add(s1, c1, zi * zi);
+    acc_add(s2, c2, zi * ri);
+  }
+  block_reduce_pair(s1, c1);
+  __syncthreads();
+  block_reduce_pair(s2, c2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = make_double2(s1, c1);
+    part[gridDim.x + blockIdx.x] = make_double2(s2, c2);
+  }
+}
+
+__global__ __launch_bounds__(1024) void cg_beta_finalize(CgParams P, const double2* __restrict__ part, int nb,
+                                                         double* __restrict__ st) {
+  if (st[kDone] != 0.0) return;
+  double zz, czz, zr, czr;
+  merge_partials(part, nb, zz, czz);
+  __syncthreads();
+  merge_partials(part + nb, nb, zr, czr);
+  if (threadIdx.x != 0) return;
+  const double dp = sqrt(fmax(zz + czz, 0.0));
+  const int its = int(st[kIts]) + 1;
+  st[kIts] = its;
+  st[kRnorm] = dp;
+  int reason = cg_converged(its, dp, st[kRnorm0], P);
+  if (reason) {
+    st[kReason] = reason;
+    st[kDone] = 1.0;
+    return;
+  }
+  const double beta_old = st[kBeta];
+  const double beta = zr + czr;
+  st[kBetaOld] = beta_old;
+  st[kBeta] = beta;
+  if (beta < 0.0) {
+    st[kReason] = kDivIndefPc;
+    st[kDone] = 1.0;
+    return;
+  }
+  st[kRatio] = beta / beta_old;
+}
+
+constexpr int kVecBlocks = 1024;
+
+int lanes_for(int n, int64_t nnz) {
+  const double avg = n > 0 ? double(nnz) / n : 1.0;
+  int l = 1;
+  while (l < 64 && l < avg) l *= 2;
+  return l;
+}
+
+template <typename VT, int LPR>
+void launch_cg_spmv(const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
+                    double* w, const double* st, double2* part, int nbs, hipStream_t s) {
+  hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, st, part);
+}
+
+template <typename VT>
+void cg_spmv(int lpr, const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
+             double* w, const double* st, double2* part, int nbs, hipStream_t s) {
+  switch (lpr) {
+    case 1: launch_cg_spmv<VT, 1>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 2: launch_cg_spmv<VT, 2>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 4: launch_cg_spmv<VT, 4>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 8: launch_cg_spmv<VT, 8>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 16: launch_cg_spmv<VT, 16>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 32: launch_cg_spmv<VT, 32>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    default: launch_cg_spmv<VT, 64>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+  }
+}
+
+int64_t spmv_blocks(int n, int lpr) { return (int64_t(n) * lpr + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+extern "C" {
+
+int64_t fne_pcg_workspace_bytes(int32_t n, int64_t nnz) {
+  if (n < 0 || nnz < 0) return FNE_EINVAL;
+  const int lpr = lanes_for(n, nnz);
+  int64_t nparts = spmv_blocks(n, lpr);
+  if (nparts < 2 * kVecBlocks) nparts = 2 * kVecBlocks;
+  return int64_t(sizeof(double)) * (5 * int64_t(n) + kStateLen) + int64_t(sizeof(double2)) * nparts;
+}
+
+int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+            const double* inv_diag, const double* b, double* x, double rtol, double atol, double dtol,
+            int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+            int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  if (n < 0 || !indptr || !indices || !vals || !b || !x || !work || max_it < 0) {
+    fne_set_error("fne_pcg: invalid arguments");
+    return FNE_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nnz64 = 0;
+  {
+    int32_t last = 0;
+    if (n > 0 && hipMemcpyAsync(&last, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      fne_set_error("fne_pcg: reading indptr failed");
+      return FNE_EHIP;
+    }
+    FNE_HIP_CHECK(hipStreamSynchronize(s));
+    nnz64 = last;
+  }
+  if (work_bytes < fne_pcg_workspace_bytes(n, nnz64)) {
+    fne_set_error("fne_pcg: workspace too small");
+    return FNE_EINVAL;
+  }
+  if (check_every < 1) check_every = 1;
+  const int lpr = lanes_for(n, nnz64);
+  const int nbs = int(spmv_blocks(n, lpr));
+  double* r = reinterpret_cast<double*>(work);
+  double* z = r + n;
+  double* w = z + n;
+  double* pa = w + n;
+  double* pb = pa + n;
+  double* st = pb + n;
+  double2* part = reinterpret_cast<double2*>(st + kStateLen);
+  CgParams P{n, indptr, indices, inv_diag, rtol, atol, dtol};
+  const int nbv = n > 0 ? int(std::min<int64_t>(kVecBlocks, (int64_t(n) + kBlock - 1) / kBlock)) : 1;
+  hipLaunchKernelGGL(cg_init_kernel, dim3(nbv), dim3(kBlock), 0, s, P, b, x, r, z, pa, part);
+  hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
+  FNE_LAUNCH_CHECK();
+  double hst[kStateLen];
+  int it = 0;
+  bool done = false;
+  while (!done) {
+    FNE_HIP_CHECK(hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, s));
+    FNE_HIP_CHECK(hipStreamSynchronize(s));
+    if (hst[kDone] != 0.0 || it >= max_it) break;
+    const int upto = std::min<int>(max_it, it + check_every);
+    for (; it < upto; ++it) {
+      double* p_old = (it & 1) ? pb : pa;
+      double* p_new = (it & 1) ? pa : pb;
+      if (precision == FNE_FP32)
+        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, st, part, nbs, s);
+      else
+        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, st, part, nbs, s);
+      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, part, nbs, st);
+      hipLaunchKernelGGL(cg_update_kernel, dim3(nbv), dim3(kBlock), 0, s, P, p_new, w, x, r, z, st, part);
+      hipLaunchKernelGGL(cg_beta_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
+    }
+    FNE_LAUNCH_CHECK();
+  }
+  const int its = int(hst[kIts]);
+  int reason = int(hst[kReason]);
+  if (hst[kDone] == 0.0) reason = -3;  // KSP_DIVERGED_ITS
+  if (its_out) *its_out = its;
+  if (reason_out) *reason_out = reason;
+  if (rnorm_out) *rnorm_out = hst[kRnorm];
+  return FNE_OK;
+}
+
+}  // extern "C"

@@ -173,9 +173,17 @@ __device__ __forceinline__ mc_f4 mfma16(float a, float b, mc_f4 c) {
 __device__ __forceinline__ mc_d4 mfma16(double a, double b, mc_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// Result layout of the 16x16x4 products: lane l holds D[row][l % 16] in slot r, with
+// row = 4 (l / 16) + r for the FP32 form and row = l / 16 + 4 r for the FP64 form.
 template <typename T> struct Vec4;
-template <> struct Vec4<float> { typedef mc_f4 type; };
-template <> struct Vec4<double> { typedef mc_d4 type; };
+template <> struct Vec4<float> {
+  typedef mc_f4 type;
+  static __device__ __forceinline__ int row(int kk, int r) { return 4 * kk + r; }
+};
+template <> struct Vec4<double> {
+  typedef mc_d4 type;
+  static __device__ __forceinline__ int row(int kk, int r) { return kk + 4 * r; }
+};
 
 // LDS of one wavefront
 template <typename HT>
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void mc_elem_kernel(fne_plan_desc d, int mo
         const int m = lane & 15;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int nr = 4 * kk + r;
+          const int nr = Vec4<HT>::row(kk, r);
           if (nr < NN && m < NN) {
             hb[(2 * nr) * NL + 2 * m] = axx[r];
             hb[(2 * nr) * NL + 2 * m + 1] = axy[r];

@@ -1,0 +1,182 @@
+"""End-to-end studies on one GPU: learned warm start and time to tolerance.
+
+``he_warmstart_study`` runs the rotating Neo-Hookean beam (``solve_he_load_steps``) for a
+set of training materials, builds (step k, step k+1) pairs from the converged histories,
+trains the per-node MLP on the device and then solves an unseen material three ways:
+previous-state start (the reference's choice, FP64 Hessian), extrapolated start and learned
+start (FP32 Hessian/SpMV).  ``slope_warmstart_study`` does the same for the Mohr-Coulomb
+strength-reduction continuation.  Both report Newton iterations, wall time and the final
+energies of each variant.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .warmstart import NodalPredictor, NodeMLP, training_pairs
+
+
+def free_node_table(freedofs: np.ndarray, perm: np.ndarray, block: int):
+    """(nodes with all components free, (n, block) reordered free indices)."""
+    fd = np.asarray(freedofs, dtype=np.int64)[np.asarray(perm, dtype=np.int64)]
+    node, comp = fd // block, fd % block
+    order = np.lexsort((comp, node))
+    node_s, comp_s = node[order], comp[order]
+    uniq, start, count = np.unique(node_s, return_index=True, return_counts=True)
+    full = count == block
+    idx = order[(start[full][:, None] + np.arange(block)[None, :])]
+    return uniq[full], idx
+
+
+def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it):
+    from .problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
+
+    p, adj, _ = build_beam_problem(nx, ny, nz, E=E, nu=nu)
+    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision=precision, ksp_type="gmres",
+                                      ksp_rtol=1e-1, ksp_max_it=ksp_max_it)
+    return p, asm
+
+
+def he_predictor(p, asm, net, mode, total_angle):
+    nodes, idx = free_node_table(p["freedofs"], asm.layout.perm, 3)
+    X = p["nodes2coord"][nodes]
+    ref = np.asarray(p["u_0_ref"], np.float64)[p["freedofs"]][asm.layout.perm]
+    offset = torch.as_tensor(ref, device=asm.device)
+    scale = float(np.abs(p["nodes2coord"]).max())
+    return NodalPredictor(idx, X, state_offset=offset, disp_scale=scale, load_scale=total_angle, net=net, mode=mode)
+
+
+def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
+                       total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=100, hidden=128, layers=4,
+                       epochs=60, lr=2e-3, seed=0, newton_kwargs=None, log=print):
+    from .problems.hyperelasticity import solve_he_load_steps
+
+    dims = [10] + [hidden] * layers + [3]
+    net = NodeMLP(dims, seed=seed)
+    feats, targs = [], []
+    t_gen = time.perf_counter()
+    for (E, nu) in train_materials:
+        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it)
+        pred = he_predictor(p, asm, None, "extrapolate", total_angle)
+        holder = {}
+        recs, _ = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
+                                      newton_kwargs=newton_kwargs, on_step=lambda r, h: holder.update(h=h))
+        hist = holder.get("h", [])
+        x, t = training_pairs(pred, hist)
+        feats.append(x)
+        targs.append(t)
+        log(f"train run E={E:.3g} nu={nu:.3f}: steps={len(recs)} nit={sum(r['nit'] for r in recs)} pairs={x.shape[0]}")
+        asm.cleanup()
+    t_gen = time.perf_counter() - t_gen
+    X = torch.cat(feats)
+    T = torch.cat(targs)
+    t0 = time.perf_counter()
+    losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"trained on {X.shape[0]} node samples: loss {losses[0]:.3e} -> {losses[-1]:.3e} in {t_train:.1f}s")
+    out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1],
+           "train_time_s": t_train, "data_gen_time_s": t_gen, "variants": {}}
+    E, nu = eval_material
+    for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
+                             ("learned_fp32", "learned", "fp32")):
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it)
+        pred = he_predictor(p, asm, net, mode, total_angle)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        recs, x = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
+                                      newton_kwargs=newton_kwargs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out["variants"][name] = {
+            "newton_its": int(sum(r["nit"] for r in recs)),
+            "linear_its": int(sum(r["linear_iters"] for r in recs)),
+            "wall_s": wall,
+            "final_energy": recs[-1]["energy"],
+            "all_converged": all("onverged" in r["message"] for r in recs),
+            "per_step_nit": [r["nit"] for r in recs],
+        }
+        log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s E={recs[-1]['energy']:.12e}")
+        asm.cleanup()
+    ref = out["variants"]["previous_fp64"]
+    for v in out["variants"].values():
+        v["energy_rel_diff_vs_fp64"] = abs(v["final_energy"] - ref["final_energy"]) / abs(ref["final_energy"])
+        v["speedup_vs_previous_fp64"] = ref["wall_s"] / v["wall_s"]
+    return out
+
+
+SLOPE_GEOMETRY = dict(x1=20.0, x2=20.0, x3=20.0, y1=10.0, y2=10.0)  # 60 x 20 m, height 10, slope 1:2
+SLOPE_MATERIAL = dict(E=20000.0, nu=0.49, c0=6.0, phi_deg=45.0, psi_deg=0.0, gamma=20.0)
+
+
+def _slope_assembler(h, degree, material, precision, ksp_max_it):
+    from .problems.slope_stability import SlopeStabilityReorderedElementAssembler, build_slope_problem
+
+    p, _, _ = build_slope_problem(h, degree=degree, geometry=SLOPE_GEOMETRY, material=material, with_adjacency=False)
+    p["phi_deg_raw"] = float(p["phi_deg"])
+    asm = SlopeStabilityReorderedElementAssembler(p, None, None, hessian_precision=precision, ksp_type="cg",
+                                                  ksp_rtol=1e-3, ksp_max_it=ksp_max_it)
+    return p, asm
+
+
+def slope_predictor(p, asm, net, mode, lam_scale=1.0):
+    nodes, idx = free_node_table(p["freedofs"], asm.layout.perm, 2)
+    X = p["nodes"][nodes]
+    offset = torch.zeros(asm.layout.n_free, dtype=torch.float64, device=asm.device)
+    return NodalPredictor(idx, X, state_offset=offset, disp_scale=0.1, load_scale=lam_scale, net=net, mode=mode)
+
+
+def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=1.0, eval_h=1.0 / 7.0, degree=4,
+                          ksp_max_it=2000, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0, newton_kwargs=None,
+                          log=print):
+    from .continuation import elastic_initial_guess, run_lambda_continuation
+
+    dims = [7] + [hidden] * layers + [2]
+    net = NodeMLP(dims, seed=seed)
+    feats, targs = [], []
+    for mat in train_materials:
+        p, asm = _slope_assembler(train_h, degree, mat, "fp32", ksp_max_it)
+        x, _ = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
+        pred = slope_predictor(p, asm, None, "extrapolate")
+        holder = {}
+        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_max_it=ksp_max_it,
+                                       newton_kwargs=newton_kwargs, on_step=lambda r, h: holder.update(h=h))
+        xs, ts = training_pairs(pred, holder.get("h", []))
+        feats.append(xs)
+        targs.append(ts)
+        log(f"train run {mat}: nit={sum(r['nit'] for r in recs)} pairs={xs.shape[0]}")
+        asm.cleanup()
+    X = torch.cat(feats)
+    T = torch.cat(targs)
+    losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
+    log(f"trained on {X.shape[0]} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "variants": {}}
+    for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
+                             ("learned_fp32", "learned", "fp32")):
+        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, ksp_max_it)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, el_its = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
+        pred = slope_predictor(p, asm, net, mode)
+        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_max_it=ksp_max_it,
+                                       newton_kwargs=newton_kwargs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out["variants"][name] = {
+            "newton_its": int(sum(r["nit"] for r in recs)),
+            "linear_its": int(sum(r["linear_iters"] for r in recs)) + el_its,
+            "wall_s": wall,
+            "energies": [r["energy"] for r in recs],
+            "all_converged": all("onverged" in r["message"] for r in recs),
+            "per_step_nit": [r["nit"] for r in recs],
+        }
+        log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s")
+        asm.cleanup()
+    ref = out["variants"]["previous_fp64"]
+    for v in out["variants"].values():
+        v["energy_rel_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(v["energies"], ref["energies"]))
+        v["speedup_vs_previous_fp64"] = ref["wall_s"] / v["wall_s"]
+    return out

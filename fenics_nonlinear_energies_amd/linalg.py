@@ -134,6 +134,7 @@ class KSP:
         self.reason = 0
         self.rnorm = float("nan")
         self.pc_setup_count = 0
+        self.check_every = 16  # device CG: iterations enqueued between polls of the stop flag
 
     # PETSc-style configuration -----------------------------------------------------------
     def setType(self, t: str) -> None:
@@ -214,8 +215,31 @@ class KSP:
         else:
             raise ValueError(f"unsupported ksp_type {self.ksp_type!r} (cg | gmres | stcg)")
 
-    # KSPCG, preconditioned-norm convergence --------------------------------------------
+    # KSPCG on the device (fne_pcg): scalars stay on the GPU, the host polls the stop flag
     def _solve_cg(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        import ctypes as C
+
+        A = self.A
+        lib = native.lib()
+        nbytes = int(lib.fne_pcg_workspace_bytes(A.n, A.nnz))
+        if nbytes < 0:
+            native.check(-1, "fne_pcg_workspace_bytes")
+        if getattr(self, "_cg_work", None) is None or self._cg_work.numel() < nbytes:
+            self._cg_work = torch.empty(nbytes, dtype=torch.uint8, device=b.device)
+        its, reason, rnorm = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        native.check(
+            lib.fne_pcg(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data), A.fne_precision,
+                        native.ptr(self._inv_diag), native.ptr(b), native.ptr(x), self.rtol, self.atol, self.dtol,
+                        self.max_it, int(self.check_every), native.ptr(self._cg_work), nbytes, C.byref(its),
+                        C.byref(reason), C.byref(rnorm), native.stream_ptr()),
+            "fne_pcg",
+        )
+        self.its = int(its.value)
+        self.reason = int(reason.value)
+        self.rnorm = float(rnorm.value)
+
+    # KSPCG, host-driven restatement (kept for checking the device solver) ----------------
+    def _solve_cg_host(self, b: torch.Tensor, x: torch.Tensor) -> None:
         A = self.A
         x.zero_()
         r = b.clone()

@@ -170,6 +170,22 @@ int fne_spmv(int32_t n_rows, const int32_t* indptr, const int32_t* indices, cons
 int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
                  int32_t precision, double* diag, fne_stream_t stream);
 
+/*
+ * Preconditioned CG on the device (KSPCG semantics: zero initial guess, point-Jacobi left
+ * preconditioning via inv_diag (NULL = none), convergence when the preconditioned residual
+ * norm <= max(rtol*initial, atol), divergence at dtol*initial, indefinite checks).  Replaces
+ * ksp.solve with ksp_type "cg" (core/petsc/scalar_problem_driver.py:196).  Scalars stay on
+ * the device; the host reads the stop flag every check_every iterations.  work: device
+ * buffer of fne_pcg_workspace_bytes(n, nnz) bytes.  Outputs: iterations, PETSc-style
+ * converged reason (2 rtol, 3 atol, -3 max_it, -4 dtol, -8 indefinite PC, -9 nan/inf,
+ * -10 indefinite matrix) and the final preconditioned residual norm (host pointers).
+ */
+int64_t fne_pcg_workspace_bytes(int32_t n, int64_t nnz);
+int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+            const double* inv_diag, const double* b, double* x, double rtol, double atol, double dtol,
+            int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+            int32_t* reason_out, double* rnorm_out, fne_stream_t stream);
+
 /* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
 int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
 
