@@ -35,7 +35,9 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int32_t* __re
                                                       const VT* __restrict__ vals,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ y) {
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware: each XCD's workgroups take a contiguous band of rows, so the gathered x
+  // entries of a band stay in that XCD's L2
+  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int row = gtid / LPR;
   const int lane = gtid % LPR;
   double acc = 0.0;

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* _
                                                          double2* __restrict__ part) {
   if (st[kDone] != 0.0) return;
   const double ratio = st[kRatio];
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int row = gtid / LPR;
   const int lane = gtid % LPR;
   double acc = 0.0;
@@ -222,7 +222,7 @@ constexpr int kVecBlocks = 1024;
 int lanes_for(int n, int64_t nnz) {
   const double avg = n > 0 ? double(nnz) / n : 1.0;
   int l = 1;
-  while (l < 64 && l < avg) l *= 2;
+  while (l < 64 && 3 * l < avg) l *= 2;  // about three entries per lane
   return l;
 }
 

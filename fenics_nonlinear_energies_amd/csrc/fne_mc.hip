@@ -29,7 +29,7 @@ namespace {
 constexpr int kMcMaxQ = 21;  // 3 elements x 21 points fill one wavefront
 constexpr int kMcEpw = 3;    // elements per wavefront iteration
 constexpr int kMcWaves = kBlock / 64;
-constexpr int kMcRowLanes = 32;
+constexpr int kMcRowLanes = 16;  // lanes per CSR row in the row gather
 constexpr unsigned char kPosNone = 255;
 
 typedef float mc_f4 __attribute__((ext_vector_type(4)));
@@ -370,8 +370,10 @@ __global__ __launch_bounds__(kBlock) void mc_elem_kernel(fne_plan_desc d, int mo
   }
 }
 
-// 32 lanes per CSR row.  inc[it] = element * 32 + local row; pos[it*32 + c] = position of the
-// element's local column c inside the CSR row (255 = constrained).
+// 16 lanes per CSR row (two element columns each).  inc[it] = element * 32 + local row; pos[it*32 + c] = position of the
+// element's local column c inside the CSR row (255 = constrained).  The row's items are read
+// in one load (lane j holds item j), then up to 8 items' element rows and positions are
+// fetched together before they are added, in item order, into the LDS copy of the row.
 template <int NL, typename HT, bool WG, bool WH>
 __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, const HT* __restrict__ hbuf,
                                                                const double* __restrict__ gbuf,
@@ -379,29 +381,62 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
                                                                double* __restrict__ g, HT* __restrict__ vals) {
   extern __shared__ unsigned char mc_smem[];
   constexpr int kRows = kBlock / kMcRowLanes;
+  constexpr int kPf = 8;
   const int cap = (d.max_row_len + 1) & ~1;
   HT* sRow = reinterpret_cast<HT*>(mc_smem) + (threadIdx.x / kMcRowLanes) * cap;
-  const int r = blockIdx.x * kRows + threadIdx.x / kMcRowLanes;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = blk * kRows + threadIdx.x / kMcRowLanes;
   const int c = threadIdx.x & (kMcRowLanes - 1);
   if (r >= d.n_free) return;  // no block-level barrier below
   const int k0 = d.indptr[r];
   const int len = d.indptr[r + 1] - k0;
   const unsigned char* pos = reinterpret_cast<const unsigned char*>(d.pitem);
+  const unsigned int* inc = reinterpret_cast<const unsigned int*>(d.inc);
   if (WH)
     for (int j = c; j < len; j += kMcRowLanes) sRow[j] = HT(0);
   __builtin_amdgcn_wave_barrier();
   double gacc = 0.0;
   const int i0 = d.inc_ptr[r], i1 = d.inc_ptr[r + 1];
-  for (int it = i0; it < i1; ++it) {
-    const unsigned int item = reinterpret_cast<const unsigned int*>(d.inc)[it];
-    const size_t e = item >> 5;
-    const int lr = int(item & 31u);
-    if (WH && c < NL) {
-      const unsigned char p = pos[size_t(it) * 32 + c];
-      if (p != kPosNone) sRow[p] += hbuf[e * (NL * NL) + lr * NL + c];
+  for (int base = i0; base < i1; base += kMcRowLanes) {
+    const int nb = min(kMcRowLanes, i1 - base);
+    const unsigned int mine = c < nb ? inc[base + c] : 0u;
+    if (WG) {
+      const double gv = c < nb ? gbuf[size_t(mine >> 5) * 32 + (mine & 31u)] : 0.0;
+      for (int j = 0; j < nb; ++j) {
+        const double t = __shfl(gv, j, kMcRowLanes);
+        if (c == 0) gacc += t;
+      }
     }
-    if (WG && c == 0) gacc += gbuf[e * 32 + lr];
-    __builtin_amdgcn_wave_barrier();
+    if (WH) {
+      constexpr int kCols = (NL + kMcRowLanes - 1) / kMcRowLanes;  // element columns per lane
+      for (int j0 = 0; j0 < nb; j0 += kPf) {
+        HT v[kPf][kCols];
+        unsigned char pp[kPf][kCols];
+#pragma unroll
+        for (int jj = 0; jj < kPf; ++jj) {
+          const int j = j0 + jj;
+          const unsigned int item = __shfl(mine, j < nb ? j : 0, kMcRowLanes);
+#pragma unroll
+          for (int q = 0; q < kCols; ++q) {
+            const int col = c + q * kMcRowLanes;
+            v[jj][q] = HT(0);
+            pp[jj][q] = kPosNone;
+            if (j < nb && col < NL) {
+              pp[jj][q] = pos[size_t(base + j) * 32 + col];
+              v[jj][q] = hbuf[size_t(item >> 5) * (NL * NL) + (item & 31u) * NL + col];
+            }
+          }
+        }
+#pragma unroll
+        for (int jj = 0; jj < kPf; ++jj) {
+#pragma unroll
+          for (int q = 0; q < kCols; ++q) {
+            if (pp[jj][q] != kPosNone) sRow[pp[jj][q]] += v[jj][q];
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+      }
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   if (WH)

@@ -31,11 +31,11 @@ def free_node_table(freedofs: np.ndarray, perm: np.ndarray, block: int):
     return uniq[full], idx
 
 
-def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it):
+def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it, ksp_type="cg"):
     from .problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
 
     p, adj, _ = build_beam_problem(nx, ny, nz, E=E, nu=nu)
-    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision=precision, ksp_type="gmres",
+    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision=precision, ksp_type=ksp_type,
                                       ksp_rtol=1e-1, ksp_max_it=ksp_max_it)
     return p, asm
 
@@ -50,8 +50,8 @@ def he_predictor(p, asm, net, mode, total_angle):
 
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
-                       total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=100, hidden=128, layers=4,
-                       epochs=60, lr=2e-3, seed=0, newton_kwargs=None, log=print):
+                       total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", hidden=128,
+                       layers=4, epochs=60, lr=2e-3, seed=0, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -59,7 +59,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     feats, targs = [], []
     t_gen = time.perf_counter()
     for (E, nu) in train_materials:
-        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it)
+        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it, ksp_type)
         pred = he_predictor(p, asm, None, "extrapolate", total_angle)
         holder = {}
         recs, _ = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
@@ -83,7 +83,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     E, nu = eval_material
     for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
                              ("learned_fp32", "learned", "fp32")):
-        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it)
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it, ksp_type)
         pred = he_predictor(p, asm, net, mode, total_angle)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
