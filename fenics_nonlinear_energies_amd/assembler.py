@@ -173,6 +173,7 @@ class ReorderedElementAssemblerBase:
         t0 = time.perf_counter()
         self.A = DeviceCSR(self.layout.indptr, self.layout.indices, self.layout.n_free, self.hessian_precision, self.device)
         self.ksp = KSP(ksp_type=ksp_type, pc_type=pc_type, rtol=ksp_rtol, max_it=ksp_max_it)
+        self.ksp.near_nullspace = self._near_nullspace_reordered()
         self._setup_timings["matrix_create"] = time.perf_counter() - t0
         self._energy_out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._setup_timings["total"] = time.perf_counter() - t_total
@@ -180,6 +181,21 @@ class ReorderedElementAssemblerBase:
     # ------------------------------------------------------------------------------------
     # problem hooks
     # ------------------------------------------------------------------------------------
+    def _near_nullspace(self) -> np.ndarray | None:
+        """Near-nullspace vectors on the free DOFs (original order) for the AMG preconditioner:
+        ``params[near_nullspace_key]`` when present (``reordered_element_base.py:1062-1090``),
+        else constants for scalar problems."""
+        key = getattr(self, "near_nullspace_key", None)
+        if key and key in self.params:
+            return np.asarray(self.params[key], dtype=np.float64)
+        if self.block_size == 1:
+            return np.ones((self.layout.n_free, 1))
+        return None
+
+    def _near_nullspace_reordered(self) -> np.ndarray | None:
+        B = self._near_nullspace()
+        return None if B is None else np.ascontiguousarray(B[self.layout.perm])
+
     def _element_geometry(self) -> np.ndarray:
         """(n_geo, n_elem) FP64 element data in the kernel's order."""
         raise NotImplementedError

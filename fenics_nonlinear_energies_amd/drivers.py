@@ -32,22 +32,30 @@ NEWTON_DEFAULTS = {
 class LinearSolveContext:
     """State shared by the assemble-and-solve callbacks (PC set-up policy, records)."""
 
-    def __init__(self, assembler, *, ksp_rtol, ksp_max_it, pc_setup_on_ksp_cap=False, save_linear_timing=False):
+    def __init__(self, assembler, *, ksp_rtol, ksp_max_it, pc_setup_on_ksp_cap=False, save_linear_timing=False,
+                 keep_pc=False):
         self.asm = assembler
         self.ksp_rtol = float(ksp_rtol)
         self.ksp_max_it = int(ksp_max_it)
         self.pc_setup_on_ksp_cap = bool(pc_setup_on_ksp_cap)
-        self.force_pc_setup_next = True
+        # keep_pc: with pc_setup_on_ksp_cap, carry an existing preconditioner over from the
+        # previous solve (e.g. the previous load step) instead of rebuilding it first
+        self.keep_pc = bool(keep_pc)
+        self.force_pc_setup_next = not (self.keep_pc and self._has_pc())
         self.save_linear_timing = bool(save_linear_timing)
         self.records: list[dict] = []
         self.iters: list[int] = []
+
+    def _has_pc(self) -> bool:
+        ksp = self.asm.ksp
+        return getattr(ksp, "_amg", None) is not None or getattr(ksp, "_inv_diag", None) is not None
 
     def reset(self, ksp_rtol=None, ksp_max_it=None) -> None:
         if ksp_rtol is not None:
             self.ksp_rtol = float(ksp_rtol)
         if ksp_max_it is not None:
             self.ksp_max_it = int(ksp_max_it)
-        self.force_pc_setup_next = True
+        self.force_pc_setup_next = not (self.keep_pc and self._has_pc())
         self.records = []
         self.iters = []
 
@@ -112,6 +120,7 @@ def solve_newton(
     first_direction_fn=None,
     first_step_laplacian=False,
     verbose=False,
+    keep_pc=False,
     **newton_kwargs,
 ):
     """Run Newton (with the reference's retry schedule) from device vector ``x`` in place.
@@ -125,7 +134,7 @@ def solve_newton(
     ls = tuple(opts.pop("linesearch_interval"))
     ctx = LinearSolveContext(
         assembler, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
-        save_linear_timing=save_linear_timing,
+        save_linear_timing=save_linear_timing, keep_pc=keep_pc,
     )
     if first_step_laplacian and first_direction_fn is None:
         def first_direction_fn(vec, rhs, sol):

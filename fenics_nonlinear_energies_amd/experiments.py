@@ -31,12 +31,12 @@ def free_node_table(freedofs: np.ndarray, perm: np.ndarray, block: int):
     return uniq[full], idx
 
 
-def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it, ksp_type="cg"):
+def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it, ksp_type="cg", pc_type="gamg"):
     from .problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
 
     p, adj, _ = build_beam_problem(nx, ny, nz, E=E, nu=nu)
     asm = HEReorderedElementAssembler(p, None, adj, hessian_precision=precision, ksp_type=ksp_type,
-                                      ksp_rtol=1e-1, ksp_max_it=ksp_max_it)
+                                      pc_type=pc_type, ksp_rtol=1e-1, ksp_max_it=ksp_max_it)
     return p, asm
 
 
@@ -50,8 +50,9 @@ def he_predictor(p, asm, net, mode, total_angle):
 
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
-                       total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", hidden=128,
-                       layers=4, epochs=60, lr=2e-3, seed=0, newton_kwargs=None, log=print):
+                       total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
+                       ksp_rtol=1e-1, pc_reuse=True, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0,
+                       guard=True, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -59,11 +60,12 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     feats, targs = [], []
     t_gen = time.perf_counter()
     for (E, nu) in train_materials:
-        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it, ksp_type)
+        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it, ksp_type, pc_type)
         pred = he_predictor(p, asm, None, "extrapolate", total_angle)
         holder = {}
         recs, _ = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
-                                      newton_kwargs=newton_kwargs, on_step=lambda r, h: holder.update(h=h))
+                                      ksp_rtol=ksp_rtol, pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs,
+                                      on_step=lambda r, h: holder.update(h=h))
         hist = holder.get("h", [])
         x, t = training_pairs(pred, hist)
         feats.append(x)
@@ -71,8 +73,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         log(f"train run E={E:.3g} nu={nu:.3f}: steps={len(recs)} nit={sum(r['nit'] for r in recs)} pairs={x.shape[0]}")
         asm.cleanup()
     t_gen = time.perf_counter() - t_gen
-    X = torch.cat(feats)
-    T = torch.cat(targs)
+    X, T = pred.fit_normalization(torch.cat(feats), torch.cat(targs))
+    stats = (pred.feat_mean, pred.feat_std, pred.targ_std)
     t0 = time.perf_counter()
     losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
     torch.cuda.synchronize()
@@ -83,12 +85,15 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     E, nu = eval_material
     for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
                              ("learned_fp32", "learned", "fp32")):
-        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it, ksp_type)
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it, ksp_type, pc_type)
         pred = he_predictor(p, asm, net, mode, total_angle)
+        pred.feat_mean, pred.feat_std, pred.targ_std = stats
+        if guard:
+            pred.energy_fn = lambda t, a=asm: float(a.energy_device(t).item())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         recs, x = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
-                                      newton_kwargs=newton_kwargs)
+                                      ksp_rtol=ksp_rtol, pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out["variants"][name] = {
@@ -98,6 +103,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
             "final_energy": recs[-1]["energy"],
             "all_converged": all("onverged" in r["message"] for r in recs),
             "per_step_nit": [r["nit"] for r in recs],
+            "learned_accepted": list(pred.accepted),
         }
         log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s E={recs[-1]['energy']:.12e}")
         asm.cleanup()

@@ -8,7 +8,8 @@
   ``cg`` (KSPCG), ``gmres`` (KSPGMRES, restart 30, modified Gram-Schmidt) and ``stcg``
   (Steihaug-Toint CG with a radius, as set by KSPCGSetRadius in core/petsc/trust_ksp.py).
   The preconditioner is point Jacobi (``jacobi``; zero diagonal entries are replaced by 1
-  as PETSc's PCJACOBI does) or ``none``.  PETSc's hypre/GAMG have no GPU counterpart here.
+  as PETSc's PCJACOBI does), ``none``, or ``gamg`` (alias ``hypre``): the smoothed-aggregation
+  AMG V-cycle of ``amg.py`` with the problem's near-nullspace.
 """
 
 from __future__ import annotations
@@ -136,6 +137,8 @@ class KSP:
         self.rnorm = float("nan")
         self.pc_setup_count = 0
         self.check_every = 16  # device CG: iterations enqueued between polls of the stop flag
+        self.near_nullspace = None  # (n, k) host array for pc_type "gamg" (rigid modes, constants)
+        self._amg = None
 
     # PETSc-style configuration -----------------------------------------------------------
     def setType(self, t: str) -> None:
@@ -161,14 +164,22 @@ class KSP:
         """Preconditioner set-up (Jacobi: inverse diagonal, zero entries -> 1)."""
         if self.A is None:
             raise RuntimeError("KSP.setUp called before setOperators")
+        self._amg = None if self.pc_type not in ("gamg", "hypre") else getattr(self, "_amg", None)
         if self.pc_type == "jacobi":
             d = self.A.diagonal()
             d = torch.where(d == 0.0, torch.ones_like(d), d)
             self._inv_diag = 1.0 / d
         elif self.pc_type == "none":
             self._inv_diag = None
+        elif self.pc_type in ("gamg", "hypre"):
+            from .amg import SmoothedAggregationPC
+
+            if self._amg is None:
+                self._amg = SmoothedAggregationPC(self.near_nullspace, precision=self.A.precision)
+            self._amg.setup(self.A.to_scipy(), self.A.indices.device)
+            self._inv_diag = None
         else:
-            raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | none)")
+            raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | gamg | none)")
         self.pc_setup_count += 1
 
     def getIterationNumber(self) -> int:
@@ -186,7 +197,9 @@ class KSP:
 
     # -------------------------------------------------------------------------------------
     def _pc(self, r: torch.Tensor, z: torch.Tensor) -> None:
-        if self._inv_diag is None:
+        if self._amg is not None:
+            self._amg.apply(r, z)
+        elif self._inv_diag is None:
             z.copy_(r)
         else:
             torch.mul(r, self._inv_diag, out=z)
@@ -204,7 +217,9 @@ class KSP:
         return 0
 
     def solve(self, b: DeviceVec, x: DeviceVec) -> None:
-        if self._inv_diag is None and self.pc_type == "jacobi":
+        if (self._inv_diag is None and self.pc_type == "jacobi") or (
+            self._amg is None and self.pc_type in ("gamg", "hypre")
+        ):
             self.setUp()
         t = self.ksp_type.lower()
         if t == "cg":
@@ -219,6 +234,10 @@ class KSP:
     # KSPCG on the device (fne_pcg): scalars stay on the GPU, the host polls the stop flag
     def _solve_cg(self, b: torch.Tensor, x: torch.Tensor) -> None:
         import ctypes as C
+
+        if self._amg is not None:  # multigrid preconditioner: host-driven iteration
+            self._solve_cg_host(b, x)
+            return
 
         A = self.A
         lib = native.lib()

@@ -192,7 +192,7 @@ HE_NEWTON = dict(tolf=1e-4, tolg=1e-3, tolg_rel=1e-3, maxit=100, linesearch_inte
 
 def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * np.pi, *, num_steps=None,
                         predictor=None, on_step=None, ksp_rtol: float = 1e-1, ksp_max_it: int = 30,
-                        newton_kwargs: dict | None = None):
+                        pc_setup_on_ksp_cap: bool = False, newton_kwargs: dict | None = None):
     """Rotate the right face in ``total_steps`` equal steps and solve each step with Newton.
 
     Follows the reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-359``):
@@ -226,7 +226,8 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
             x.t.copy_(guess)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, **opts)
+        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
+                           pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap, **opts)
         torch.cuda.synchronize()
         rec = {
             "step": k,

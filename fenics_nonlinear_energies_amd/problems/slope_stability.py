@@ -246,8 +246,17 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
                          reorder_mode=reorder_mode, **kw)
         self._constitutive_mode = 0
 
-    def _coords_for_reordering(self):
-        return np.asarray(self.params["nodes"], dtype=np.float64)
+    def _near_nullspace(self) -> np.ndarray:
+        """Plane rigid-body modes on the free DOFs (``build_near_nullspace_modes``, ``mesh.py:802-813``)."""
+        X = np.asarray(self.params["nodes"], dtype=np.float64)
+        n = X.shape[0]
+        R = np.zeros((2 * n, 3))
+        R[0::2, 0] = 1.0
+        R[1::2, 1] = 1.0
+        c = X.mean(axis=0)
+        R[0::2, 2] = -(X[:, 1] - c[1])
+        R[1::2, 2] = X[:, 0] - c[0]
+        return R[np.asarray(self.params["freedofs"], dtype=np.int64)]
 
     def _element_geometry(self) -> np.ndarray:
         K = np.asarray(self.params["K"], dtype=np.float64)

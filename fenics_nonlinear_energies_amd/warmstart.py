@@ -182,6 +182,9 @@ class NodalPredictor:
         self.net = net
         self.mode = str(mode)
         self._dev = None
+        self.feat_mean = self.feat_std = self.targ_std = None
+        self.energy_fn = None  # optional: accept the learned guess only if its energy is lower
+        self.accepted = []
 
     def _to(self, t):
         if self._dev is None:
@@ -189,12 +192,25 @@ class NodalPredictor:
             self._X = torch.as_tensor(self.coords, dtype=torch.float32, device=t.device)
         return self._dev
 
-    def features(self, u_k: torch.Tensor, u_km1: torch.Tensor, load_next: float) -> torch.Tensor:
+    def raw_features(self, u_k: torch.Tensor, u_km1: torch.Tensor, load_next: float) -> torch.Tensor:
         idx = self._to(u_k)
         dk = (u_k - self.offset)[idx].float() / self.disp_scale
         dkm1 = (u_km1 - self.offset)[idx].float() / self.disp_scale
         lp = torch.full((idx.shape[0], 1), float(load_next) / self.load_scale, device=u_k.device)
         return torch.cat((self._X / self.coord_scale, dk, dkm1, lp), dim=1).contiguous()
+
+    def features(self, u_k, u_km1, load_next) -> torch.Tensor:
+        f = self.raw_features(u_k, u_km1, load_next)
+        if self.feat_mean is not None:
+            f = (f - self.feat_mean) / self.feat_std
+        return f.contiguous()
+
+    def fit_normalization(self, raw_x: torch.Tensor, raw_t: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+        """Standardise inputs (per feature) and targets (per component) from training data."""
+        self.feat_mean = raw_x.mean(dim=0)
+        self.feat_std = raw_x.std(dim=0).clamp_min(1e-12)
+        self.targ_std = raw_t.std(dim=0).clamp_min(1e-12)
+        return (raw_x - self.feat_mean) / self.feat_std, raw_t / self.targ_std
 
     @property
     def coord_scale(self) -> float:
@@ -219,16 +235,24 @@ class NodalPredictor:
             return guess
         idx = self._to(u_k)
         corr = self.net.forward(self.features(u_k, u_km1, load_next))
-        guess = guess.clone()
-        guess[idx] += self.disp_scale * corr.double()
-        return guess
+        if self.targ_std is not None:
+            corr = corr * self.targ_std
+        learned = guess.clone()
+        learned[idx] += self.disp_scale * corr.double()
+        if self.energy_fn is not None:
+            # Newton's own merit function decides: keep the extrapolation if the network's
+            # guess has the higher energy
+            ok = self.energy_fn(learned) <= self.energy_fn(guess)
+            self.accepted.append(bool(ok))
+            return learned if ok else guess
+        return learned
 
 
 def training_pairs(predictor: NodalPredictor, history) -> tuple[torch.Tensor, torch.Tensor]:
-    """(features, targets) of every (k-1, k, k+1) triple of a converged history."""
+    """(raw features, raw targets) of every (k-1, k, k+1) triple of a converged history."""
     xs, ts = [], []
     for j in range(1, len(history) - 1):
         (_, u_km1), (_, u_k), (load_next, u_next) = history[j - 1], history[j], history[j + 1]
-        xs.append(predictor.features(u_k, u_km1, load_next))
+        xs.append(predictor.raw_features(u_k, u_km1, load_next))
         ts.append(predictor.target(u_k, u_km1, u_next))
     return torch.cat(xs), torch.cat(ts)

@@ -1,0 +1,49 @@
+"""Smoothed-aggregation AMG preconditioner (pc_type "gamg") on the GPU.
+
+CG with the V-cycle reaches the solution of a direct solve (rel. 1e-6 at rtol 1e-10) in far
+fewer iterations than point Jacobi, for an elasticity Hessian (rigid-body near-nullspace)
+and a scalar p-Laplace Hessian (constant near-nullspace).
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from fenics_nonlinear_energies_amd.problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
+from fenics_nonlinear_energies_amd.problems.plaplace import PLaplaceReorderedElementAssembler, build_plaplace_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(asm, pc, rtol, b):
+    ksp = asm.ksp
+    ksp.pc_type = pc
+    ksp._amg = None
+    ksp._inv_diag = None
+    ksp.setOperators(asm.A)
+    ksp.setTolerances(rtol=rtol, max_it=20000)
+    ksp.setUp()
+    x = b.duplicate()
+    ksp.solve(b, x)
+    return x, ksp.getIterationNumber()
+
+
+@pytest.mark.parametrize("kind", ["he", "plaplace"])
+def test_gamg_cg_matches_direct_solve(require_gpu, kind):
+    if kind == "he":
+        p, adj, u0 = build_beam_problem(40, 4, 4)
+        asm = HEReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+        u = u0 + 1e-5 * np.random.default_rng(0).standard_normal(u0.size)
+        x0 = asm.create_vec(u[asm.layout.perm])
+    else:
+        p, adj, _ = build_plaplace_problem(96)
+        asm = PLaplaceReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+        x0 = asm.create_vec(np.random.default_rng(0).random(asm.layout.n_free))
+    asm.assemble_hessian(x0)
+    b = asm.create_vec(np.random.default_rng(1).standard_normal(asm.layout.n_free))
+    xg, its_g = _solve(asm, "gamg", 1e-10, b)
+    _, its_j = _solve(asm, "jacobi", 1e-10, b)
+    ref = spla.spsolve(asm.A.to_scipy().tocsc(), b.array)
+    assert np.linalg.norm(xg.array - ref) <= 1e-6 * np.linalg.norm(ref)
+    assert its_g * 4 < its_j, (its_g, its_j)
+    asm.cleanup()
