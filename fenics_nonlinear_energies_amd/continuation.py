@@ -60,6 +60,7 @@ def run_lambda_continuation(
     on_step: Callable[[dict, list], None] | None = None,
     ksp_rtol: float = 1e-3,
     ksp_max_it: int = 2000,
+    pc_setup_on_ksp_cap: bool = True,
     newton_kwargs: dict | None = None,
 ) -> list[dict]:
     """Solve the reduced-strength problem for each lambda, warm-starting from the history.
@@ -85,7 +86,8 @@ def run_lambda_continuation(
             x.t.copy_(guess)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, **opts)
+        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
+                           keep_pc=pc_setup_on_ksp_cap, **opts)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         rec = {
@@ -103,6 +105,8 @@ def run_lambda_continuation(
         }
         records.append(rec)
         history.append((float(lam), x.t.clone()))
+        if predictor is not None and hasattr(predictor, "accepted") and used_pred and predictor.accepted:
+            rec["learned_accepted"] = bool(predictor.accepted[-1])
         if on_step is not None:
             on_step(rec, history)
     return records

@@ -118,13 +118,13 @@ SLOPE_GEOMETRY = dict(x1=20.0, x2=20.0, x3=20.0, y1=10.0, y2=10.0)  # 60 x 20 m,
 SLOPE_MATERIAL = dict(E=20000.0, nu=0.49, c0=6.0, phi_deg=45.0, psi_deg=0.0, gamma=20.0)
 
 
-def _slope_assembler(h, degree, material, precision, ksp_max_it):
+def _slope_assembler(h, degree, material, precision, ksp_max_it, pc_type="gamg"):
     from .problems.slope_stability import SlopeStabilityReorderedElementAssembler, build_slope_problem
 
     p, _, _ = build_slope_problem(h, degree=degree, geometry=SLOPE_GEOMETRY, material=material, with_adjacency=False)
     p["phi_deg_raw"] = float(p["phi_deg"])
     asm = SlopeStabilityReorderedElementAssembler(p, None, None, hessian_precision=precision, ksp_type="cg",
-                                                  ksp_rtol=1e-3, ksp_max_it=ksp_max_it)
+                                                  pc_type=pc_type, ksp_rtol=1e-3, ksp_max_it=ksp_max_it)
     return p, asm
 
 
@@ -135,28 +135,30 @@ def slope_predictor(p, asm, net, mode, lam_scale=1.0):
     return NodalPredictor(idx, X, state_offset=offset, disp_scale=0.1, load_scale=lam_scale, net=net, mode=mode)
 
 
-def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=1.0, eval_h=1.0 / 7.0, degree=4,
-                          ksp_max_it=2000, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0, newton_kwargs=None,
-                          log=print):
+def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.0, eval_h=0.5, degree=4,
+                          ksp_max_it=2000, ksp_rtol=1e-3, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0,
+                          guard=True, newton_kwargs=None, log=print):
+    """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (FP32)."""
     from .continuation import elastic_initial_guess, run_lambda_continuation
 
     dims = [7] + [hidden] * layers + [2]
     net = NodeMLP(dims, seed=seed)
     feats, targs = [], []
+    pred = None
     for mat in train_materials:
         p, asm = _slope_assembler(train_h, degree, mat, "fp32", ksp_max_it)
         x, _ = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
         pred = slope_predictor(p, asm, None, "extrapolate")
         holder = {}
-        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_max_it=ksp_max_it,
+        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
                                        newton_kwargs=newton_kwargs, on_step=lambda r, h: holder.update(h=h))
         xs, ts = training_pairs(pred, holder.get("h", []))
         feats.append(xs)
         targs.append(ts)
         log(f"train run {mat}: nit={sum(r['nit'] for r in recs)} pairs={xs.shape[0]}")
         asm.cleanup()
-    X = torch.cat(feats)
-    T = torch.cat(targs)
+    X, T = pred.fit_normalization(torch.cat(feats), torch.cat(targs))
+    stats = (pred.feat_mean, pred.feat_std, pred.targ_std)
     losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
     log(f"trained on {X.shape[0]} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "variants": {}}
@@ -167,7 +169,10 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=1.
         t0 = time.perf_counter()
         x, el_its = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
         pred = slope_predictor(p, asm, net, mode)
-        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_max_it=ksp_max_it,
+        pred.feat_mean, pred.feat_std, pred.targ_std = stats
+        if guard:
+            pred.energy_fn = lambda t, a=asm: float(a.energy_device(t).item())
+        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
                                        newton_kwargs=newton_kwargs)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -178,6 +183,8 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=1.
             "energies": [r["energy"] for r in recs],
             "all_converged": all("onverged" in r["message"] for r in recs),
             "per_step_nit": [r["nit"] for r in recs],
+            "learned_accepted": list(pred.accepted),
+            "n_free": int(asm.layout.n_free),
         }
         log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s")
         asm.cleanup()
