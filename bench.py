@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=256)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--no-secondary", action="store_true", help="skip the P4 slope and tet beam assembly lines")
     ap.add_argument(
         "--copy-calib-mib", type=int, default=0,
         help="before warm-up, run 3 device copies of this many MiB (counter calibration reference)",
@@ -71,6 +72,63 @@ def cpu_baseline(n_cells: int, budget_s: float) -> dict:
         "sample": f"GL {n_cells}x{n_cells} P1 ({ne} elements), numpy FP64 element energy+gradient+Hessian "
         f"with np.add.at scatter into CSR, median of {len(times)} repeats",
     }
+
+
+def _time_ms(fn, reps=20):
+    import torch
+
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(torch.cuda.current_stream())
+    for _ in range(reps):
+        fn()
+    b.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def secondary_workloads() -> list[dict]:
+    """Fused Newton-step assembly and SpMV on the P4 slope (C5 size) and the tet beam (C3 size)."""
+    import numpy as np
+
+    from fenics_nonlinear_energies_amd.problems import slope_stability as S
+    from fenics_nonlinear_energies_amd.problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
+
+    out = []
+    geo = dict(x1=20.0, x2=20.0, x3=20.0, y1=10.0, y2=10.0)
+    mat = dict(E=20000.0, nu=0.49, c0=6.0, phi_deg=45.0, psi_deg=0.0, gamma=20.0)
+    p, _, _ = S.build_slope_problem(1.0 / 7.0, degree=4, geometry=geo, material=mat, with_adjacency=False)
+    asm = S.SlopeStabilityReorderedElementAssembler(p, None, None, hessian_precision="fp32")
+    X = p["nodes"]
+    v = np.zeros(2 * X.shape[0])
+    v[0::2] = 1e-3 * X[:, 1]
+    v[1::2] = -5e-4 * X[:, 1]
+    x = asm.create_vec(v[p["freedofs"]][asm.layout.perm])
+    g, y = x.duplicate(), x.duplicate()
+    ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t))
+    sp = _time_ms(lambda: asm.A.mult(x, y))
+    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
+    out.append({"workload": "Mohr-Coulomb P4 slope 60x20 m, h=1/7 (plastic state)", "n_elem": ne, "n_free": nf,
+                "nnz": int(nnz), "assembly_ms": ms, "elem_per_s": ne / (ms * 1e-3), "spmv_ms": sp,
+                "spmv_GBs": (8 * nnz + 20 * nf) / (sp * 1e-3) / 1e9,
+                "mfma_TFLOPs": ne * 112.9e3 / (ms * 1e-3) / 1e12})
+    asm.cleanup()
+    del asm
+    p, adj, u0 = build_beam_problem(160, 8, 8)
+    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision="fp32")
+    x = asm.create_vec((u0 + 1e-4 * np.random.default_rng(0).standard_normal(u0.size))[asm.layout.perm])
+    g, y = x.duplicate(), x.duplicate()
+    ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t))
+    sp = _time_ms(lambda: asm.A.mult(x, y))
+    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
+    b_asm = ne * (4 * 12 + 8 * 13) + 8 * asm.dirichlet_full.size + 8 * nf + 4 * nnz
+    out.append({"workload": "Neo-Hookean beam 160x8x8 hexes x 6 tets", "n_elem": ne, "n_free": nf, "nnz": int(nnz),
+                "assembly_ms": ms, "elem_per_s": ne / (ms * 1e-3), "assembly_GBs": b_asm / (ms * 1e-3) / 1e9,
+                "spmv_ms": sp, "spmv_GBs": (8 * nnz + 20 * nf) / (sp * 1e-3) / 1e9})
+    asm.cleanup()
+    return out
 
 
 def main():
@@ -193,6 +251,8 @@ def main():
         },
     }
     asm.cleanup()
+    if not args.no_secondary:
+        out["secondary"] = secondary_workloads()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_n, args.cpu_budget_s)
     print(json.dumps(out), flush=True)

@@ -370,55 +370,78 @@ __global__ __launch_bounds__(kBlock) void mc_elem_kernel(fne_plan_desc d, int mo
   }
 }
 
-// 16 lanes per CSR row (two element columns each).  inc[it] = element * 32 + local row; pos[it*32 + c] = position of the
-// element's local column c inside the CSR row (255 = constrained).  The row's items are read
-// in one load (lane j holds item j), then up to 8 items' element rows and positions are
-// fetched together before they are added, in item order, into the LDS copy of the row.
+// Row gather.  A group of 16 lanes owns kMcRowBatch consecutive CSR rows; lane c handles the
+// element columns c and c + 16.  inc[it] = element * 32 + local row lists the incidences of
+// every row (rows consecutive, elements ascending); pos[it*32 + c] is the position of the
+// element's local column c inside the row (255 = constrained).  The incidences of the whole
+// batch are read in one load (lane j holds item j), up to 8 items' element rows and positions
+// are fetched together, and then added in item order into LDS copies of the rows, which are
+// written once.  Every CSR entry therefore sums its contributions in a fixed order.
+constexpr int kMcRowBatch = 4;
 template <int NL, typename HT, bool WG, bool WH>
 __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, const HT* __restrict__ hbuf,
                                                                const double* __restrict__ gbuf,
                                                                const double* __restrict__ f,
                                                                double* __restrict__ g, HT* __restrict__ vals) {
   extern __shared__ unsigned char mc_smem[];
-  constexpr int kRows = kBlock / kMcRowLanes;
+  constexpr int kL = kMcRowLanes;
+  constexpr int kRB = kMcRowBatch;
+  constexpr int kRowsBlk = (kBlock / kL) * kRB;
   constexpr int kPf = 8;
+  constexpr int kCols = (NL + kL - 1) / kL;  // element columns per lane
   const int cap = (d.max_row_len + 1) & ~1;
-  HT* sRow = reinterpret_cast<HT*>(mc_smem) + (threadIdx.x / kMcRowLanes) * cap;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int r = blk * kRows + threadIdx.x / kMcRowLanes;
-  const int c = threadIdx.x & (kMcRowLanes - 1);
-  if (r >= d.n_free) return;  // no block-level barrier below
-  const int k0 = d.indptr[r];
-  const int len = d.indptr[r + 1] - k0;
+  const int grp = threadIdx.x / kL;
+  const int c = threadIdx.x & (kL - 1);
+  HT* sRows = reinterpret_cast<HT*>(mc_smem) + size_t(grp) * kRB * cap;
+  const int r0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsBlk + grp * kRB;
+  if (r0 >= d.n_free) return;  // uniform over the 16-lane group; no block-level barrier below
+  const int nr = min(kRB, d.n_free - r0);
   const unsigned char* pos = reinterpret_cast<const unsigned char*>(d.pitem);
   const unsigned int* inc = reinterpret_cast<const unsigned int*>(d.inc);
-  if (WH)
-    for (int j = c; j < len; j += kMcRowLanes) sRow[j] = HT(0);
+  const int ip_l = c <= nr ? d.indptr[r0 + c] : 0;
+  const int ic_l = c <= nr ? d.inc_ptr[r0 + c] : 0;
+  int kb[kRB + 1], ib[kRB + 1];
+#pragma unroll
+  for (int t = 0; t <= kRB; ++t) {
+    kb[t] = __shfl(ip_l, t < nr ? t : nr, kL);
+    ib[t] = __shfl(ic_l, t < nr ? t : nr, kL);
+  }
+  if (WH) {
+#pragma unroll
+    for (int t = 0; t < kRB; ++t)
+      if (t < nr)
+        for (int j = c; j < kb[t + 1] - kb[t]; j += kL) sRows[t * cap + j] = HT(0);
+  }
   __builtin_amdgcn_wave_barrier();
-  double gacc = 0.0;
-  const int i0 = d.inc_ptr[r], i1 = d.inc_ptr[r + 1];
-  for (int base = i0; base < i1; base += kMcRowLanes) {
-    const int nb = min(kMcRowLanes, i1 - base);
+  double gacc = 0.0;  // lane t < nr: gradient of row r0 + t
+  for (int base = ib[0]; base < ib[nr]; base += kL) {
+    const int nb = min(kL, ib[nr] - base);
     const unsigned int mine = c < nb ? inc[base + c] : 0u;
+    int myslot = 0;
+#pragma unroll
+    for (int t = 1; t < kRB; ++t)
+      if (t < nr && base + c >= ib[t]) myslot = t;
     if (WG) {
       const double gv = c < nb ? gbuf[size_t(mine >> 5) * 32 + (mine & 31u)] : 0.0;
       for (int j = 0; j < nb; ++j) {
-        const double t = __shfl(gv, j, kMcRowLanes);
-        if (c == 0) gacc += t;
+        const double v = __shfl(gv, j, kL);
+        const int t = __shfl(myslot, j, kL);
+        if (c == t) gacc += v;
       }
     }
     if (WH) {
-      constexpr int kCols = (NL + kMcRowLanes - 1) / kMcRowLanes;  // element columns per lane
       for (int j0 = 0; j0 < nb; j0 += kPf) {
         HT v[kPf][kCols];
         unsigned char pp[kPf][kCols];
+        int sl[kPf];
 #pragma unroll
         for (int jj = 0; jj < kPf; ++jj) {
           const int j = j0 + jj;
-          const unsigned int item = __shfl(mine, j < nb ? j : 0, kMcRowLanes);
+          const unsigned int item = __shfl(mine, j < nb ? j : 0, kL);
+          sl[jj] = __shfl(myslot, j < nb ? j : 0, kL);
 #pragma unroll
           for (int q = 0; q < kCols; ++q) {
-            const int col = c + q * kMcRowLanes;
+            const int col = c + q * kL;
             v[jj][q] = HT(0);
             pp[jj][q] = kPosNone;
             if (j < nb && col < NL) {
@@ -431,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
         for (int jj = 0; jj < kPf; ++jj) {
 #pragma unroll
           for (int q = 0; q < kCols; ++q) {
-            if (pp[jj][q] != kPosNone) sRow[pp[jj][q]] += v[jj][q];
+            if (pp[jj][q] != kPosNone) sRows[sl[jj] * cap + pp[jj][q]] += v[jj][q];
             __builtin_amdgcn_wave_barrier();
           }
         }
@@ -439,9 +462,13 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  if (WH)
-    for (int j = c; j < len; j += kMcRowLanes) vals[k0 + j] = sRow[j];
-  if (WG && c == 0) g[r] = f ? gacc - f[r] : gacc;
+  if (WH) {
+#pragma unroll
+    for (int t = 0; t < kRB; ++t)
+      if (t < nr)
+        for (int j = c; j < kb[t + 1] - kb[t]; j += kL) vals[kb[t] + j] = sRows[t * cap + j];
+  }
+  if (WG && c < nr) g[r0 + c] = f ? gacc - f[r0 + c] : gacc;
 }
 
 // Element outputs for fne_element_eval: scratch (element-major) -> SoA outputs.
@@ -525,9 +552,13 @@ int launch_mc(fne_plan* plan, FneOp op, const FneCall& c) {
     if (rc) return rc;
   }
   if (!want_g && !want_h) return FNE_OK;
-  const int rows_per_block = kBlock / kMcRowLanes;
+  const int rows_per_block = (kBlock / kMcRowLanes) * kMcRowBatch;
   const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
   const int cap = (d.max_row_len + 1) & ~1;
+  if (want_h && size_t(rows_per_block) * cap * (fp32 ? 4 : 8) > plan->max_lds) {
+    fne_set_error("row gather LDS request exceeds the device limit");
+    return FNE_EINVAL;
+  }
   if (want_h) {
     const size_t shm = size_t(rows_per_block) * cap * (fp32 ? 4 : 8);
     if (fp32) {
