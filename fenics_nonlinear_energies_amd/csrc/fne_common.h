@@ -136,5 +136,8 @@ int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c);
 int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c);
 int fne_launch_mc(fne_plan* plan, FneOp op, const FneCall& c);
 int fne_finalize_energy(fne_plan* plan, int nblocks, double* out, hipStream_t stream);
-// bytes of the Mohr-Coulomb element-matrix scratch (and, via gbytes, of the gradient scratch)
+// bytes of the element-matrix scratch (and, via gbytes, of the gradient scratch) used by the
+// row-gather assembly (Mohr-Coulomb always, tetrahedra with FNE_FLAG_ROW_GATHER)
 size_t fne_mc_scratch_bytes(const fne_plan_desc& d, size_t* gbytes);
+// CSR rows gather their incident element rows from the scratch (n_loc 12 or 30)
+int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c);

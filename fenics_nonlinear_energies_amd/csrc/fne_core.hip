@@ -234,7 +234,9 @@ int fne_plan_create(const fne_plan_desc* desc, fne_plan** out) {
     fne_set_error("hipMalloc of reduction scratch failed");
     return FNE_ENOMEM;
   }
-  if (desc->kind == FNE_KIND_MOHR_COULOMB && desc->n_elem > 0) {
+  if ((desc->kind == FNE_KIND_MOHR_COULOMB ||
+       (desc->kind == FNE_KIND_NEOHOOKEAN && (desc->flags & FNE_FLAG_ROW_GATHER))) &&
+      desc->n_elem > 0) {
     size_t gbytes = 0;
     const size_t hbytes = fne_mc_scratch_bytes(*desc, &gbytes);
     if (hipMalloc(&p->mc_h, hbytes) != hipSuccess || hipMalloc(&p->mc_g, gbytes) != hipSuccess) {

@@ -586,6 +586,40 @@ int launch_mc(fne_plan* plan, FneOp op, const FneCall& c) {
 
 }  // namespace
 
+int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c) {
+  const fne_plan_desc& d = plan->d;
+  const bool fp32 = c.precision == FNE_FP32;
+  const int rows_per_block = (kBlock / kMcRowLanes) * kMcRowBatch;
+  const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
+  const int cap = (d.max_row_len + 1) & ~1;
+  const size_t shm = want_h ? size_t(rows_per_block) * cap * (fp32 ? 4 : 8) : 0;
+  if (shm > plan->max_lds || d.max_row_len > 254) {
+    fne_set_error("row gather: CSR rows too long for the LDS row copies");
+    return FNE_EINVAL;
+  }
+  if (d.n_free == 0 || (!want_g && !want_h)) return FNE_OK;
+  const void* hb = plan->mc_h;
+  const double* gb = plan->mc_g;
+#define RG(NL, HT, G, H)                                                                                   \
+  hipLaunchKernelGGL((mc_row_gather_kernel<NL, HT, G, H>), grid, dim3(kBlock), shm, c.stream, d, (const HT*)hb, \
+                     gb, c.f, c.g, (HT*)c.vals)
+  if (d.n_loc == 12) {
+    if (!want_h) RG(12, double, true, false);
+    else if (fp32) { if (want_g) RG(12, float, true, true); else RG(12, float, false, true); }
+    else { if (want_g) RG(12, double, true, true); else RG(12, double, false, true); }
+  } else if (d.n_loc == 30) {
+    if (!want_h) RG(30, double, true, false);
+    else if (fp32) { if (want_g) RG(30, float, true, true); else RG(30, float, false, true); }
+    else { if (want_g) RG(30, double, true, true); else RG(30, double, false, true); }
+  } else {
+    fne_set_error("row gather supports elements with 12 or 30 DOFs");
+    return FNE_EINVAL;
+  }
+#undef RG
+  FNE_LAUNCH_CHECK();
+  return FNE_OK;
+}
+
 size_t fne_mc_scratch_bytes(const fne_plan_desc& d, size_t* gbytes) {
   const size_t nl = size_t(d.n_loc);
   if (gbytes) *gbytes = size_t(d.n_elem) * 32 * sizeof(double);

@@ -269,6 +269,39 @@ __global__ __launch_bounds__(kBlock) void tet_element_kernel(fne_plan_desc d, co
 
 }  // namespace
 
+// Row-gather form (FNE_FLAG_ROW_GATHER): one thread per element writes its gradient
+// (gbuf[e][32], FP64) and 12x12 Hessian (hbuf[e][144], HT) element-major; the CSR rows then
+// gather them (fne_launch_row_gather).
+template <typename HT, bool WANT_G, bool WANT_H>
+__global__ __launch_bounds__(kBlock) void tet_store_kernel(fne_plan_desc d, const double* __restrict__ x,
+                                                           const double* __restrict__ dvals,
+                                                           double* __restrict__ gbuf, HT* __restrict__ hbuf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.n_elem) return;
+  const bool absdet = (d.flags & FNE_FLAG_ABS_DET) != 0;
+  TetElem t;
+  tet_load(d, e, x, dvals, t);
+  TetState s;
+  tet_state(t, d.param, absdet, s);
+  if (WANT_G)
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i) gbuf[size_t(e) * 32 + 3 * a + i] = s.g[a][i];
+  if (WANT_H) {
+    HT* hb = hbuf + size_t(e) * 144;
+    for (int a = 0; a < 4; ++a)
+      for (int i = 0; i < 3; ++i)
+        for (int b = 0; b < 4; ++b) {
+          const double dd = t.D[a][0] * t.D[b][0] + t.D[a][1] * t.D[b][1] + t.D[a][2] * t.D[b][2];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double h = s.s1 * s.q[b][i] * s.q[a][k] + s.s2 * s.q[a][i] * s.q[b][k];
+            if (i == k) h += s.s0 * dd;
+            hb[(3 * a + i) * 12 + 3 * b + k] = HT(h);
+          }
+        }
+  }
+}
+
 int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
   if (op == kOpEnergy) {
@@ -294,9 +327,29 @@ int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c) {
     }
     return FNE_OK;
   }
-  if (d.n_patch <= 0) return FNE_OK;
   const bool want_h = (op == kOpHessian);
   const bool want_g = (op == kOpGradient) || (c.g != nullptr);
+  if (d.flags & FNE_FLAG_ROW_GATHER) {
+    if (!plan->mc_g || !plan->mc_h || !d.inc || !d.inc_ptr || (want_h && !d.pitem)) {
+      fne_set_error("row-gather arrays missing from the tetrahedron plan");
+      return FNE_EINVAL;
+    }
+    const int nb = (d.n_elem + kBlock - 1) / kBlock;
+    if (nb > 0) {
+      if (!want_h)
+        hipLaunchKernelGGL((tet_store_kernel<double, true, false>), dim3(nb), dim3(kBlock), 0, c.stream, d, c.x,
+                           c.dvals, plan->mc_g, (double*)plan->mc_h);
+      else if (c.precision == FNE_FP32)
+        hipLaunchKernelGGL((tet_store_kernel<float, true, true>), dim3(nb), dim3(kBlock), 0, c.stream, d, c.x,
+                           c.dvals, plan->mc_g, (float*)plan->mc_h);
+      else
+        hipLaunchKernelGGL((tet_store_kernel<double, true, true>), dim3(nb), dim3(kBlock), 0, c.stream, d, c.x,
+                           c.dvals, plan->mc_g, (double*)plan->mc_h);
+      FNE_LAUNCH_CHECK();
+    }
+    return fne_launch_row_gather(plan, want_g, want_h, c);
+  }
+  if (d.n_patch <= 0) return FNE_OK;
   const size_t hsz = c.precision == FNE_FP32 ? 4 : 8;
   size_t shm = sizeof(double) * size_t(kTetStage) * size_t(d.chunk);
   if (want_h) shm += ((size_t(d.max_patch_nnz) * hsz + 7) / 8) * 8;

@@ -16,8 +16,8 @@ import numpy as np
 import scipy.sparse as sp
 
 from ..assembler import ReorderedElementAssemblerBase
-from ..native import FNE_FLAG_ABS_DET, FNE_KIND_NEOHOOKEAN
-from ..patches import build_block_plan
+from ..native import FNE_FLAG_ABS_DET, FNE_FLAG_ROW_GATHER, FNE_KIND_NEOHOOKEAN
+from ..patches import build_block_plan, build_row_incidence_plan
 
 # corner offsets (di, dj, dk) of the six tetrahedra of one hexahedron
 _HEX_TETS = (
@@ -152,6 +152,9 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
                  ksp_max_it=30, reorder_mode="block_morton", **kw):
         self.use_abs_det = bool(use_abs_det)
         kw.setdefault("rows_per_patch", None)
+        # element-major element matrices + CSR row gather: measured faster than the node-block
+        # patch schedule on the 160x8x8 (2.8x) and 320x16x16 (1.2x) beams
+        kw.setdefault("assembly_mode", "rowgather")
         super().__init__(params, comm, adjacency, ksp_type=ksp_type, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
                          reorder_mode=reorder_mode, **kw)
 
@@ -166,7 +169,10 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
         return [float(self.params["C1"]), float(self.params["D1"])]
 
     def _plan_flags(self) -> int:
-        return FNE_FLAG_ABS_DET if self.use_abs_det else 0
+        flags = FNE_FLAG_ABS_DET if self.use_abs_det else 0
+        if self.assembly_mode == "rowgather":
+            flags |= FNE_FLAG_ROW_GATHER
+        return flags
 
     def _lds_bytes_per_element(self) -> int:
         return 39 * 8
@@ -175,8 +181,10 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
         return 4 if self.hessian_precision == "fp32" else 8
 
     def _build_patch_plan(self, elem_free, rows_per_patch, chunk):
-        # pick the node count per patch so the CSR slice leaves room for >= 64 staged elements
         lay = self.layout
+        if self.assembly_mode == "rowgather":
+            return build_row_incidence_plan(elem_free, lay.indptr, lay.indices, lay.n_free)
+        # pick the node count per patch so the CSR slice leaves room for >= 64 staged elements
         row_len = np.diff(np.asarray(lay.indptr, dtype=np.int64))
         max_row = int(row_len.max(initial=1))
         budget = self.lds_budget_bytes() - 64 * self._lds_bytes_per_element() - 64

@@ -19,10 +19,11 @@ def _state(p, u0, seed, amp=2e-4):
     return u0 + amp * np.random.default_rng(seed).standard_normal(u0.size)
 
 
+@pytest.mark.parametrize("amode", ["staged", "rowgather"])
 @pytest.mark.parametrize("absdet", [False, True])
-def test_he_energy_gradient_hessian_fp64(require_gpu, absdet):
+def test_he_energy_gradient_hessian_fp64(require_gpu, absdet, amode):
     p, adj, u0 = build_beam_problem(16, 3, 3)
-    asm = HEReorderedElementAssembler(p, None, adj, use_abs_det=absdet)
+    asm = HEReorderedElementAssembler(p, None, adj, use_abs_det=absdet, assembly_mode=amode)
     u = _state(p, u0, 1)
     lay = asm.layout
     x = asm.create_vec(u[lay.perm])
@@ -37,9 +38,10 @@ def test_he_energy_gradient_hessian_fp64(require_gpu, absdet):
     asm.cleanup()
 
 
-def test_he_fp32_hessian_and_rotated_face(require_gpu):
+@pytest.mark.parametrize("amode", ["staged", "rowgather"])
+def test_he_fp32_hessian_and_rotated_face(require_gpu, amode):
     p, adj, u0 = build_beam_problem(20, 2, 2)
-    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision="fp32")
+    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision="fp32", assembly_mode=amode)
     u0_rot = rotate_right_face_from_reference(p["u_0_ref"], p["nodes2coord"], 0.3, p["right_nodes"])
     asm.update_dirichlet(u0_rot)
     pr = dict(p)

@@ -229,9 +229,11 @@ def test_host_plans_fit_kernel_limits(prec):
             lds = 24 * plan.chunk + hsz * (((plan.max_patch_nnz + 1) & ~1) + 6 * plan.chunk)
             assert lds <= 160 * 1024  # gfx950 workgroup LDS limit
             assert int(np.max(np.diff(lay_indptr := asm.layout.indptr))) <= 30
-        else:
-            lds = asm._lds_slice_bytes_per_entry() * plan.max_patch_nnz + asm._lds_bytes_per_element() * plan.chunk
-            assert lds <= asm.lds_budget_bytes() and plan.chunk >= 64
+        else:  # tetrahedra: row-gather plan (16 lanes x 4 rows per group, rows <= 254 entries)
+            assert plan.n_patch == 0 and plan.pitem.shape == (plan.inc.size, 32)
+            assert plan.max_row_len <= 254
+            assert 64 * ((plan.max_row_len + 1) & ~1) * 8 <= 160 * 1024
+            assert plan.inc_ptr[-1] == plan.inc.size
         assert plan.rows_per_patch <= 256
         assert np.all(np.diff(plan.prow) <= 256)
         assert asm._host_codes.shape == (p["elems"].shape[0], p["elems"].shape[1])
