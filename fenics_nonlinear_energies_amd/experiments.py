@@ -92,8 +92,10 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
             pred.energy_fn = lambda t, a=asm: float(a.energy_device(t).item())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        recs, x = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
-                                      ksp_rtol=ksp_rtol, pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs)
+        recs, x = solve_he_load_steps(
+            asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it, ksp_rtol=ksp_rtol,
+            pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs,
+            on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out["variants"][name] = {
