@@ -14,9 +14,10 @@ Galerkin products P^T A P, recursively, until the coarse problem is small enough
 Cholesky factorisation.
 
 Apply (device): one symmetric V-cycle -- two damped-Jacobi sweeps before and after the
-coarse correction -- with every operator product on the CSR SpMV kernel (``fne_spmv``) and
-the coarsest solve as a dense triangular-factor product.  The cycle is a fixed symmetric
-positive-definite linear operator, so it preconditions CG.
+coarse correction -- with every operator product, the coarsest dense inverse included, on
+the CSR SpMV kernel (``fne_spmv``).  The cycle is recorded once per set-up as a HIP graph
+and replayed.  It is a fixed symmetric positive-definite linear operator, so it
+preconditions CG.
 """
 
 from __future__ import annotations
@@ -92,8 +93,10 @@ class SmoothedAggregationPC:
     """``pc_type="gamg"``: smoothed-aggregation AMG V-cycle on the device."""
 
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | None = None, max_coarse: int = 1200,
-                 max_levels: int = 8, sweeps: int = 2, precision: str = "fp64"):
+                 max_levels: int = 8, sweeps: int = 2, precision: str = "fp64", use_graph: bool = True):
         self.B = None if near_nullspace is None else np.asarray(near_nullspace, dtype=np.float64)
+        self.use_graph = bool(use_graph)
+        self._graph = None
         self.agg_dofs = agg_dofs
         self.max_coarse = int(max_coarse)
         self.max_levels = int(max_levels)
@@ -152,12 +155,35 @@ class SmoothedAggregationPC:
         else:
             Linv = sla.solve_triangular(L, np.eye(L.shape[0]), lower=True)
             self.coarse = torch.as_tensor(np.ascontiguousarray(Linv)).to(device)
+        # the coarse inverse Linv^T Linv as a dense CSR operator: one SpMV kernel, no BLAS call
+        Ainv = Linv.T @ Linv
+        self.coarse_op = _DevMat(sp.csr_matrix(Ainv), device, "fp64")
         self.levels = levels
         self.setup_count += 1
+        self._graph = None
+        if self.use_graph and device is not None and torch.device(device).type == "cuda":
+            self._capture(n, device)
+
+    def _capture(self, n: int, device) -> None:
+        """Record one V-cycle as a HIP graph (static input/output buffers): the cycle is
+        launch-bound on the coarse levels, and a replay issues all its kernels at once."""
+        self._g_in = torch.zeros(n, dtype=torch.float64, device=device)
+        self._g_out = torch.zeros(n, dtype=torch.float64, device=device)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                self._g_out.copy_(self._cycle(0, self._g_in))
+        torch.cuda.current_stream().wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._g_out.copy_(self._cycle(0, self._g_in))
+        self._graph = g
 
     def _coarse_solve(self, r: torch.Tensor) -> torch.Tensor:
-        # A^-1 r = Linv^T (Linv r)
-        return self.coarse.t().mv(self.coarse.mv(r))
+        y = torch.empty(self.coarse_op.n_rows, dtype=r.dtype, device=r.device)
+        self.coarse_op.mult(r, y)
+        return y
 
     def _cycle(self, lvl: int, r: torch.Tensor) -> torch.Tensor:
         if lvl == len(self.levels):
@@ -182,7 +208,12 @@ class SmoothedAggregationPC:
         return x
 
     def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
-        z.copy_(self._cycle(0, r))
+        if self._graph is not None:
+            self._g_in.copy_(r)
+            self._graph.replay()
+            z.copy_(self._g_out)
+        else:
+            z.copy_(self._cycle(0, r))
 
     def summary(self) -> dict:
         return {"levels": len(self.levels) + 1, "sizes": [L["n"] for L in self.levels] + [int(self.coarse.shape[0])]}

@@ -47,3 +47,29 @@ def test_gamg_cg_matches_direct_solve(require_gpu, kind):
     assert np.linalg.norm(xg.array - ref) <= 1e-6 * np.linalg.norm(ref)
     assert its_g * 4 < its_j, (its_g, its_j)
     asm.cleanup()
+
+
+def test_vcycle_graph_replays_match_eager(require_gpu):
+    """The V-cycle recorded as a HIP graph gives bitwise the eager result on every replay
+    (several replays, with unrelated eager work in between)."""
+    import torch
+
+    p, adj, _ = build_plaplace_problem(64)
+    asm = PLaplaceReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+    asm.assemble_hessian(asm.create_vec(np.random.default_rng(0).random(asm.layout.n_free)))
+    ksp = asm.ksp
+    ksp.setOperators(asm.A)
+    ksp.setUp()
+    pc = ksp._amg
+    assert pc._graph is not None
+    rng = np.random.default_rng(5)
+    for k in range(4):
+        r = torch.as_tensor(rng.standard_normal(asm.layout.n_free)).cuda()
+        eager = pc._cycle(0, r.clone())
+        junk = torch.randn(10000, device="cuda").cumsum(0)  # unrelated eager work
+        z = torch.empty_like(r)
+        pc.apply(r, z)
+        torch.cuda.synchronize()
+        assert torch.equal(z, eager), k
+        assert torch.isfinite(junk).all()
+    asm.cleanup()
