@@ -4,7 +4,8 @@ Stands in for the reference's PETSc GAMG / hypre preconditioners with near-nulls
 (``use_near_nullspace``, ``hyperelasticity/jax_petsc/solver.py:140-217``;
 ``slope_stability/jax_petsc/solver.py:44-63``), which have no GPU counterpart here.
 
-Set-up (host, numpy/scipy, once per operator): aggregates are consecutive runs of
+Set-up (host, numpy/scipy, once per operator; level operators and transfers are stored in
+the operator's precision, FP32 in the mixed-precision mode): aggregates are consecutive runs of
 ``agg_dofs`` unknowns in the assembler's reordered numbering -- the Morton order makes them
 compact spatial blobs, so no graph aggregation pass is needed; the tentative prolongator
 is the per-aggregate QR of the near-nullspace vectors (rigid-body modes for elasticity,
@@ -131,8 +132,8 @@ class SmoothedAggregationPC:
             Ac = 0.5 * (Ac + Ac.T)
             levels.append({
                 "A": _DevMat(A, device, self.precision),
-                "P": _DevMat(P, device, "fp64"),
-                "R": _DevMat(P.T.tocsr(), device, "fp64"),
+                "P": _DevMat(P, device, self.precision),
+                "R": _DevMat(P.T.tocsr(), device, self.precision),
                 "dinv": torch.as_tensor(dinv).to(device),
                 "omega": omega,
                 "n": A.shape[0],
