@@ -2,26 +2,40 @@
 
 Stands in for the reference's PETSc GAMG / hypre preconditioners with near-nullspace
 (``use_near_nullspace``, ``hyperelasticity/jax_petsc/solver.py:140-217``;
-``slope_stability/jax_petsc/solver.py:44-63``), which have no GPU counterpart here.
+``slope_stability/jax_petsc/solver.py:44-63``), which have no GPU counterpart here.  Like
+PETSc's, the hierarchy is rebuilt from the current Hessian at every set-up.
 
-Set-up (host, numpy/scipy, once per operator; level operators and transfers are stored in
-the operator's precision, FP32 in the mixed-precision mode): aggregates are consecutive runs of
-``agg_dofs`` unknowns in the assembler's reordered numbering -- the Morton order makes them
-compact spatial blobs, so no graph aggregation pass is needed; the tentative prolongator
-is the per-aggregate QR of the near-nullspace vectors (rigid-body modes for elasticity,
-constants for scalar problems); it is smoothed once with damped Jacobi,
-P = (I - omega D^-1 A) P_tent with omega = 4 / (3 rho(D^-1 A)); coarse operators are
-Galerkin products P^T A P, recursively, until the coarse problem is small enough for a dense
-Cholesky factorisation.
+Hierarchy: aggregates are consecutive runs of ``agg_dofs`` unknowns in the assembler's
+reordered numbering -- the (isotropic) Morton order makes them compact spatial blobs, so no
+graph aggregation pass is needed.  By default a fine-level aggregate holds 16 nodes' worth of
+unknowns (16 k for k near-nullspace vectors) and a coarse-level one 8 coarse nodes (8 k): on
+the Neo-Hookean beam this keeps the operator complexity near 1.4 at about 9 CG iterations
+per decade-1 solve.  The tentative prolongator is the per-aggregate QR of the
+near-nullspace vectors (rigid-body modes for elasticity, constants for scalar problems);
+it is smoothed once with damped Jacobi, P = (I - omega D^-1 A) P_tent with
+omega = 4 / (3 rho(D^-1 A)); coarse operators are Galerkin products P^T A P, recursively,
+until the coarse problem is small enough for a dense inverse.
+
+Set-up in two phases.  Symbolic (host, scipy, once per Hessian pattern): the patterns of
+P, A P, P^T and P^T A P on every level, the transpose permutations and the diagonal
+positions.  Numeric (device, every set-up): the per-aggregate QR of the near-nullspace
+(host, batched numpy -- the near-nullspace may follow the current state), the spectral radius
+by power iteration, and the three products per level on the fixed-pattern product kernel
+(``fne_spgemm_numeric``); only the coarsest operator goes to the host for its dense inverse.
+Level operators and transfers are kept in FP64 for the products and copied to the
+operator's precision (FP32 in the mixed-precision mode) for the cycle.
 
 Apply (device): one symmetric V-cycle -- two damped-Jacobi sweeps before and after the
 coarse correction -- with every operator product, the coarsest dense inverse included, on
-the CSR SpMV kernel (``fne_spmv``).  The cycle is recorded once per set-up as a HIP graph
-and replayed.  It is a fixed symmetric positive-definite linear operator, so it
-preconditions CG.
+the CSR SpMV kernel (``fne_spmv``); the finest level uses the live Hessian.  The cycle is
+recorded as a HIP graph after the first set-up and replayed; later set-ups rewrite its
+buffers in place.  It is a fixed symmetric positive-definite linear operator between
+set-ups, so it preconditions CG.
 """
 
 from __future__ import annotations
+
+import time
 
 import numpy as np
 import scipy.linalg as sla
@@ -31,23 +45,26 @@ import torch
 from . import native
 
 
-class _DevMat:
-    """CSR operator on the device (rows may differ from columns)."""
+def _lanes_for(avg: float) -> int:
+    lpr = 1
+    while lpr < 64 and 3 * lpr < avg:
+        lpr *= 2
+    return lpr
 
-    def __init__(self, A: sp.csr_matrix, device, precision: str):
-        A = A.tocsr()
-        A.sort_indices()
-        self.n_rows = int(A.shape[0])
-        self.indptr = torch.as_tensor(A.indptr.astype(np.int32)).to(device)
-        self.indices = torch.as_tensor(A.indices.astype(np.int32)).to(device)
+
+class _DevMat:
+    """CSR operator on the device (rows may differ from columns); ``data`` is rewritten in place."""
+
+    def __init__(self, indptr: np.ndarray, indices: np.ndarray, n_cols: int, device, precision: str):
+        self.n_rows = int(np.asarray(indptr).size - 1)
+        self.n_cols = int(n_cols)
+        self.nnz = int(np.asarray(indices).size)
+        self.indptr = torch.as_tensor(np.asarray(indptr, dtype=np.int32)).to(device)
+        self.indices = torch.as_tensor(np.asarray(indices, dtype=np.int32)).to(device)
         dt = torch.float32 if precision == "fp32" else torch.float64
-        self.data = torch.as_tensor(A.data.astype(np.float64)).to(device=device, dtype=dt)
+        self.data = torch.zeros(self.nnz, dtype=dt, device=device)
         self.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
-        avg = A.nnz / max(1, self.n_rows)
-        lpr = 1
-        while lpr < 64 and 3 * lpr < avg:
-            lpr *= 2
-        self.lpr = lpr
+        self.lpr = _lanes_for(self.nnz / max(1, self.n_rows))
 
     def mult(self, x: torch.Tensor, y: torch.Tensor) -> None:
         native.check(
@@ -58,43 +75,89 @@ class _DevMat:
         )
 
 
-def _spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, its: int = 15, seed: int = 0) -> float:
-    x = np.random.default_rng(seed).standard_normal(A.shape[0])
-    lam = 1.0
-    for _ in range(its):
-        y = dinv * (A @ x)
-        lam = float(np.linalg.norm(y) / max(np.linalg.norm(x), 1e-300))
-        x = y / max(np.linalg.norm(y), 1e-300)
-    return lam * 1.05
+def _spgemm(a_indptr, a_indices, a_data, b: _DevMat, b_data: torch.Tensor, c: _DevMat, c_data: torch.Tensor):
+    """c_data <- (A B) on C's pattern (fne_spgemm_numeric); A given by its CSR arrays."""
+    n_rows = int(a_indptr.numel() - 1)
+    prec = native.FNE_FP32 if a_data.dtype == torch.float32 else native.FNE_FP64
+    native.check(
+        native.lib().fne_spgemm_numeric(n_rows, native.ptr(a_indptr), native.ptr(a_indices), native.ptr(a_data), prec,
+                                        native.ptr(b.indptr), native.ptr(b.indices), native.ptr(b_data),
+                                        native.ptr(c.indptr), native.ptr(c.indices), native.ptr(c_data),
+                                        native.stream_ptr()),
+        "fne_spgemm_numeric",
+    )
+
+
+def _pattern(M: sp.csr_matrix) -> sp.csr_matrix:
+    """Structure of M with all-ones values (sorted columns)."""
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    return sp.csr_matrix((np.ones(M.nnz), M.indices.copy(), M.indptr.copy()), shape=M.shape)
+
+
+def _pattern_product(A: sp.csr_matrix, B: sp.csr_matrix) -> sp.csr_matrix:
+    # all-positive values: no entry of the product cancels, so this is the full structure
+    return _pattern(_pattern(A) @ _pattern(B))
+
+
+def _transpose_perm(M: sp.csr_matrix) -> np.ndarray:
+    """perm with (M^T).data == M.data[perm] for M^T in sorted CSR."""
+    T = sp.csr_matrix((np.arange(M.nnz, dtype=np.float64), M.indices, M.indptr), shape=M.shape).T.tocsr()
+    T.sort_indices()
+    return T.data.astype(np.int64)
+
+
+def _positions(M: sp.csr_matrix, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
+    """Index into M.data of each (row, col) entry (all must be in M's sorted pattern)."""
+    key_m = np.repeat(np.arange(M.shape[0], dtype=np.int64), np.diff(M.indptr)) * M.shape[1] + M.indices
+    key = np.asarray(rows, dtype=np.int64) * M.shape[1] + np.asarray(cols, dtype=np.int64)
+    pos = np.searchsorted(key_m, key)
+    if key.size and (np.any(pos >= key_m.size) or np.any(key_m[np.minimum(pos, key_m.size - 1)] != key)):
+        raise ValueError("entry outside the pattern")
+    return pos
+
+
+def _aggregate_sizes(n: int, agg: int) -> np.ndarray:
+    na, rem = divmod(n, agg)
+    return np.array([agg] * na + ([rem] if rem else []), dtype=np.int64)
 
 
 def tentative_prolongator(B: np.ndarray, agg_dofs: int):
-    """Block-diagonal P_tent (n x n_agg k) with orthonormal blocks and the coarse near-nullspace."""
+    """Block-diagonal P_tent (n x sum_a min(rows_a, k), CSR) and the coarse near-nullspace.
+
+    Aggregate a holds rows [a*agg, (a+1)*agg); its block is the Q factor of those rows of B
+    (batched numpy QR) and its coarse near-nullspace rows are the R factor.
+    """
     n, k = B.shape
-    n_agg = -(-n // agg_dofs)
-    rows, cols, vals = [], [], []
-    Bc = np.zeros((n_agg * k, k))
-    for a in range(n_agg):
-        lo, hi = a * agg_dofs, min(n, (a + 1) * agg_dofs)
-        Q, R = np.linalg.qr(B[lo:hi])
-        kk = Q.shape[1]
-        ii = np.arange(lo, hi)
-        for m in range(kk):
-            rows.append(ii)
-            cols.append(np.full(ii.size, a * k + m))
-            vals.append(Q[:, m])
-        Bc[a * k:a * k + kk] = R
-    P = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n_agg * k))
-    # drop coarse columns of rank-deficient aggregates (all-zero columns)
-    keep = np.flatnonzero(np.asarray(abs(P).sum(axis=0)).ravel() > 0)
-    return P[:, keep].tocsr(), Bc[keep]
+    sizes = _aggregate_sizes(n, agg_dofs)
+    na_full = n // agg_dofs
+    vals, Bc = [], []
+    if na_full:
+        Q, R = np.linalg.qr(B[: na_full * agg_dofs].reshape(na_full, agg_dofs, k))
+        vals.append(Q.reshape(-1))
+        Bc.append(R.reshape(-1, k))
+    if sizes.size > na_full:
+        Q, R = np.linalg.qr(B[na_full * agg_dofs:])
+        vals.append(Q.reshape(-1))
+        Bc.append(R)
+    kk = np.minimum(sizes, k)
+    col0 = np.concatenate(([0], np.cumsum(kk)[:-1]))
+    row_k = np.repeat(kk, sizes)
+    indptr = np.concatenate(([0], np.cumsum(row_k))).astype(np.int64)
+    start = np.repeat(np.repeat(col0, sizes), row_k)  # first coarse column of each entry's aggregate
+    indices = start + np.arange(indptr[-1]) - np.repeat(indptr[:-1], row_k)
+    Pt = sp.csr_matrix((np.concatenate(vals), indices, indptr), shape=(n, int(kk.sum())))
+    return Pt, np.concatenate(Bc, axis=0)
 
 
 class SmoothedAggregationPC:
     """``pc_type="gamg"``: smoothed-aggregation AMG V-cycle on the device."""
 
-    def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | None = None, max_coarse: int = 1200,
-                 max_levels: int = 8, sweeps: int = 2, precision: str = "fp64", use_graph: bool = True):
+    POWER_ITS = 12
+
+    def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
+                 max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
+                 use_graph: bool = True):
         self.B = None if near_nullspace is None else np.asarray(near_nullspace, dtype=np.float64)
         self.use_graph = bool(use_graph)
         self._graph = None
@@ -103,68 +166,175 @@ class SmoothedAggregationPC:
         self.max_levels = int(max_levels)
         self.sweeps = int(sweeps)
         self.precision = precision
-        self.levels = []
+        self.levels: list[dict] = []
         self.setup_count = 0
+        self.setup_time = 0.0
+        self.symbolic_time = 0.0
+        self._key = None
+        self._A0 = None
 
-    def setup(self, A_host: sp.csr_matrix, device) -> None:
-        A = A_host.tocsr().astype(np.float64)
-        n = A.shape[0]
-        B = self.B if self.B is not None else np.ones((n, 1))
-        k = B.shape[1]
-        agg = int(self.agg_dofs or (8 * k if k > 1 else 8))
+    # ---------------------------------------------------------------------------------------
+    def _aggs(self, k: int) -> tuple:
+        if self.agg_dofs is None:
+            return (16 * k, 8 * k) if k > 1 else (8,)
+        return tuple(int(a) for a in np.atleast_1d(self.agg_dofs))
+
+    def _symbolic(self, A0, device, k: int) -> None:
+        """Patterns of every level's products (host, once per Hessian pattern)."""
+        t0 = time.perf_counter()
+        n = A0.n
+        a_indices = A0.indices.cpu().numpy()
+        A_pat = _pattern(sp.csr_matrix((np.ones(A0.nnz), a_indices, A0.indptr.cpu().numpy()), shape=(n, n)))
+        if not np.array_equal(A_pat.indices, a_indices):
+            raise ValueError("the Hessian's column indices must be sorted within each row")
+        aggs = self._aggs(k)
+        prec = self.precision
         levels = []
-        cache = getattr(self, "_ptent", None)
-        if cache is None:
-            cache = self._ptent = []  # tentative prolongators depend on the near-nullspace only
         for lvl in range(self.max_levels):
-            d = A.diagonal().copy()
-            d[d == 0.0] = 1.0
-            dinv = 1.0 / d
-            if A.shape[0] <= self.max_coarse:
+            nl = A_pat.shape[0]
+            if nl <= self.max_coarse:
                 break
-            if lvl >= len(cache):
-                cache.append(tentative_prolongator(B, max(agg, B.shape[1])))
-            Pt, Bc = cache[lvl]
-            rho = _spectral_radius_dinv_a(A, dinv)
-            omega = 4.0 / (3.0 * rho)
-            P = (Pt - omega * sp.diags(dinv) @ (A @ Pt)).tocsr()
-            Ac = (P.T @ (A @ P)).tocsr()
-            Ac = 0.5 * (Ac + Ac.T)
+            agg = max(aggs[min(lvl, len(aggs) - 1)], k)
+            Pt, _ = tentative_prolongator(np.ones((nl, k)), agg)
+            nc = Pt.shape[1]
+            if nc >= nl:
+                break
+            P_pat = _pattern_product(A_pat, Pt)
+            AP_pat = _pattern_product(A_pat, P_pat)
+            R_pat = _pattern(P_pat.T.tocsr())
+            Ac_pat = _pattern_product(R_pat, AP_pat)
+            if Ac_pat.shape != (nc, nc):
+                raise RuntimeError("coarse pattern has the wrong shape")
+            pt_rows = np.repeat(np.arange(nl), np.diff(Pt.indptr))
+            diag = _positions(A_pat, np.arange(nl), np.arange(nl))
             levels.append({
-                "A": _DevMat(A, device, self.precision),
-                "P": _DevMat(P, device, self.precision),
-                "R": _DevMat(P.T.tocsr(), device, self.precision),
-                "dinv": torch.as_tensor(dinv).to(device),
-                "omega": omega,
-                "n": A.shape[0],
+                "n": nl,
+                "nc": nc,
+                "agg": agg,
+                # FP64 masters for the products
+                "Pt": _DevMat(Pt.indptr, Pt.indices, nc, device, "fp64"),
+                "P64": _DevMat(P_pat.indptr, P_pat.indices, nc, device, "fp64"),
+                "AP64": _DevMat(AP_pat.indptr, AP_pat.indices, nc, device, "fp64"),
+                "R64": _DevMat(R_pat.indptr, R_pat.indices, nl, device, "fp64"),
+                "Ac64": _DevMat(Ac_pat.indptr, Ac_pat.indices, nc, device, "fp64"),
+                # transfers of the cycle, in the operator's precision
+                "P": _DevMat(P_pat.indptr, P_pat.indices, nc, device, prec),
+                "R": _DevMat(R_pat.indptr, R_pat.indices, nl, device, prec),
+                "ptpos": torch.as_tensor(_positions(P_pat, pt_rows, Pt.indices)).to(device),
+                "p_rows": torch.as_tensor(np.repeat(np.arange(nl), np.diff(P_pat.indptr))).to(device),
+                "tperm_p": torch.as_tensor(_transpose_perm(P_pat)).to(device),
+                "tperm_ac": torch.as_tensor(_transpose_perm(Ac_pat)).to(device),
+                "diag": torch.as_tensor(diag).to(device),
+                "wd": torch.zeros(nl, dtype=torch.float64, device=device),
+                "v0": torch.as_tensor(np.random.default_rng(lvl).standard_normal(nl)).to(device),
             })
-            if Ac.shape[0] >= A.shape[0]:
-                A = Ac
-                break
-            A, B = Ac, Bc
-        # coarsest level: dense Cholesky (shifted by a tiny multiple of the trace if singular)
-        Ad = A.toarray()
+            A_pat = Ac_pat
+        # the cycle's copy of each coarse operator, in the operator's precision
+        for i in range(1, len(levels)):
+            src = levels[i - 1]["Ac64"]
+            levels[i]["A"] = _DevMat(src.indptr.cpu().numpy(), src.indices.cpu().numpy(), levels[i]["n"], device,
+                                     prec)
+        nc = A_pat.shape[0]
+        self._coarse_pat = A_pat
+        cols = np.tile(np.arange(nc), nc)
+        self.coarse_op = _DevMat(np.arange(0, nc * nc + 1, nc), cols, nc, device, "fp64")
+        self._coarse_n = nc
+        self.levels = levels
+        self.symbolic_time += time.perf_counter() - t0
+
+    # ---------------------------------------------------------------------------------------
+    def _tentative_values(self, B: np.ndarray) -> list:
+        """P_tent values of every level (host QR) from the near-nullspace of the finest."""
+        out = []
+        for L in self.levels:
+            Pt, Bc = tentative_prolongator(B, L["agg"])
+            if Pt.nnz != L["Pt"].nnz:
+                raise RuntimeError("near-nullspace changed the aggregate structure")
+            out.append(Pt.data)
+            B = Bc
+        return out
+
+    @staticmethod
+    def _key_of(A0, k: int) -> tuple:
+        return (id(A0), A0.nnz, A0.indices.data_ptr(), A0.data.data_ptr(), A0.data.dtype, int(k))
+
+    def setup(self, A0, device=None, near_nullspace: np.ndarray | None = None) -> None:
+        """(Re)build the hierarchy from the device Hessian ``A0`` (a ``DeviceCSR``).
+
+        ``near_nullspace`` (n x k host array) replaces the one given at construction for this
+        and later set-ups (e.g. rigid-body modes at the current configuration).
+        """
+        t_setup = time.perf_counter()
+        device = A0.indices.device if device is None else device
+        if near_nullspace is not None:
+            self.B = np.asarray(near_nullspace, dtype=np.float64)
+        n = A0.n
+        B = self.B if self.B is not None else np.ones((n, 1))
+        if B.shape[0] != n:
+            raise ValueError(f"near-nullspace has {B.shape[0]} rows for an operator of size {n}")
+        key = self._key_of(A0, B.shape[1])
+        if key != self._key:
+            self._graph = None
+            self._symbolic(A0, device, B.shape[1])
+            self._key = key
+        self._A0 = A0
+        pt_vals = self._tentative_values(B)
+        a_ptr, a_idx, a_dat = A0.indptr, A0.indices, A0.data
+        for lvl, L in enumerate(self.levels):
+            d = A0.diagonal() if lvl == 0 else a_dat[L["diag"]]
+            d = torch.where(d == 0.0, torch.ones_like(d), d)
+            dinv = 1.0 / d
+            # rho(D^-1 A) by power iteration on the device (no host synchronisation)
+            v = L["v0"] / torch.linalg.vector_norm(L["v0"])
+            y = torch.empty_like(v)
+            lam = None
+            for _ in range(self.POWER_ITS):
+                if lvl == 0:
+                    A0.mult_tensor(v, y)
+                else:
+                    self.levels[lvl - 1]["Ac64"].mult(v, y)
+                y.mul_(dinv)
+                lam = torch.linalg.vector_norm(y)
+                v = y / lam
+            omega = 4.0 / (3.0 * 1.05 * lam)
+            L["wd"].copy_(omega * dinv)
+            # P = P_tent - omega D^-1 (A P_tent)
+            Pt, P64 = L["Pt"], L["P64"]
+            Pt.data.copy_(torch.as_tensor(pt_vals[lvl]))
+            _spgemm(a_ptr, a_idx, a_dat, Pt, Pt.data, P64, P64.data)
+            P64.data.mul_(-omega * dinv[L["p_rows"]])
+            P64.data.index_add_(0, L["ptpos"], Pt.data)
+            # A P, R = P^T, A_c = R (A P), symmetrised
+            AP64, R64, Ac64 = L["AP64"], L["R64"], L["Ac64"]
+            _spgemm(a_ptr, a_idx, a_dat, P64, P64.data, AP64, AP64.data)
+            torch.index_select(P64.data, 0, L["tperm_p"], out=R64.data)
+            _spgemm(R64.indptr, R64.indices, R64.data, AP64, AP64.data, Ac64, Ac64.data)
+            Ac64.data.copy_(0.5 * (Ac64.data + Ac64.data[L["tperm_ac"]]))
+            L["P"].data.copy_(P64.data)
+            L["R"].data.copy_(R64.data)
+            if lvl + 1 < len(self.levels):
+                self.levels[lvl + 1]["A"].data.copy_(Ac64.data)
+            a_ptr, a_idx, a_dat = Ac64.indptr, Ac64.indices, Ac64.data
+        # coarsest level: dense inverse on the host (Cholesky; eigen-pseudo-inverse if singular)
+        nc, cp = self._coarse_n, self._coarse_pat
+        Ad = sp.csr_matrix((a_dat.double().cpu().numpy(), cp.indices, cp.indptr), shape=(nc, nc)).toarray()
         Ad = 0.5 * (Ad + Ad.T)
         try:
-            L = sla.cholesky(Ad, lower=True)
+            Lc = sla.cholesky(Ad, lower=True)
+            Linv = sla.solve_triangular(Lc, np.eye(nc), lower=True)
+            Ainv = Linv.T @ Linv
         except np.linalg.LinAlgError:
-            shift = 1e-12 * np.trace(Ad) / max(1, Ad.shape[0])
+            shift = 1e-12 * np.trace(Ad) / max(1, nc)
             w, V = np.linalg.eigh(Ad)
             w = np.where(w > shift, w, np.inf)
-            Linv = (V / np.sqrt(w)).T  # pseudo-inverse factor
-            self.coarse = torch.as_tensor(np.ascontiguousarray(Linv)).to(device)
-        else:
-            Linv = sla.solve_triangular(L, np.eye(L.shape[0]), lower=True)
-            self.coarse = torch.as_tensor(np.ascontiguousarray(Linv)).to(device)
-        # the coarse inverse Linv^T Linv as a dense CSR operator: one SpMV kernel, no BLAS call
-        Ainv = Linv.T @ Linv
-        self.coarse_op = _DevMat(sp.csr_matrix(Ainv), device, "fp64")
-        self.levels = levels
+            Ainv = (V / w) @ V.T
+        self.coarse_op.data.copy_(torch.as_tensor(np.ascontiguousarray(Ainv).ravel()))
         self.setup_count += 1
-        self._graph = None
-        if self.use_graph and device is not None and torch.device(device).type == "cuda":
+        self.setup_time += time.perf_counter() - t_setup
+        if self.use_graph and self._graph is None and torch.device(device).type == "cuda":
             self._capture(n, device)
 
+    # ---------------------------------------------------------------------------------------
     def _capture(self, n: int, device) -> None:
         """Record one V-cycle as a HIP graph (static input/output buffers): the cycle is
         launch-bound on the coarse levels, and a replay issues all its kernels at once."""
@@ -186,17 +356,23 @@ class SmoothedAggregationPC:
         self.coarse_op.mult(r, y)
         return y
 
+    def _apply_A(self, lvl: int, x: torch.Tensor, y: torch.Tensor) -> None:
+        if lvl == 0:
+            self._A0.mult_tensor(x, y)
+        else:
+            self.levels[lvl]["A"].mult(x, y)
+
     def _cycle(self, lvl: int, r: torch.Tensor) -> torch.Tensor:
         if lvl == len(self.levels):
             return self._coarse_solve(r)
         L = self.levels[lvl]
-        A, dinv, om = L["A"], L["dinv"], L["omega"]
-        x = om * dinv * r
+        wd = L["wd"]
+        x = wd * r
         t = torch.empty_like(r)
         for _ in range(self.sweeps - 1):
-            A.mult(x, t)
-            x.add_(om * dinv * (r - t))
-        A.mult(x, t)
+            self._apply_A(lvl, x, t)
+            x.add_(wd * (r - t))
+        self._apply_A(lvl, x, t)
         res = r - t
         rc = torch.empty(L["R"].n_rows, dtype=r.dtype, device=r.device)
         L["R"].mult(res, rc)
@@ -204,11 +380,17 @@ class SmoothedAggregationPC:
         L["P"].mult(xc, t)
         x.add_(t)
         for _ in range(self.sweeps):
-            A.mult(x, t)
-            x.add_(om * dinv * (r - t))
+            self._apply_A(lvl, x, t)
+            x.add_(wd * (r - t))
         return x
 
     def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        if self._A0 is None:
+            raise RuntimeError("multigrid apply before setup()")
+        if self._key_of(self._A0, self._key[-1]) != self._key:
+            # the Hessian buffers moved (e.g. a precision change): the recorded cycle would
+            # read freed memory
+            raise RuntimeError("the Hessian was reallocated since the multigrid set-up; call setup() again")
         if self._graph is not None:
             self._g_in.copy_(r)
             self._graph.replay()
@@ -216,5 +398,10 @@ class SmoothedAggregationPC:
         else:
             z.copy_(self._cycle(0, r))
 
+    def operator_complexity(self) -> float:
+        nnz = [self._A0.nnz] + [L["Ac64"].nnz for L in self.levels]
+        return float(sum(nnz) / max(1, nnz[0]))
+
     def summary(self) -> dict:
-        return {"levels": len(self.levels) + 1, "sizes": [L["n"] for L in self.levels] + [int(self.coarse.shape[0])]}
+        return {"levels": len(self.levels) + 1, "sizes": [L["n"] for L in self.levels] + [int(self._coarse_n)],
+                "operator_complexity": round(self.operator_complexity(), 3)}

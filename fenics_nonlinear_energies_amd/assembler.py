@@ -440,6 +440,7 @@ class ReorderedElementAssemblerBase:
         self._record_callback("gradient", kernel=dt, total=dt)
 
     def hessian_device(self, x: torch.Tensor, g: torch.Tensor | None = None) -> None:
+        self._hessian_state = x  # read by state-dependent preconditioner set-ups
         native.check(
             native.lib().fne_hessian(
                 self._plan,
@@ -458,6 +459,7 @@ class ReorderedElementAssemblerBase:
                                    out: torch.Tensor | None = None) -> torch.Tensor:
         """One fused pass: energy (device scalar), gradient (optional) and Hessian into ``A``."""
         out = self._energy_out if out is None else out
+        self._hessian_state = x
         native.check(
             native.lib().fne_energy_grad_hessian(
                 self._plan,

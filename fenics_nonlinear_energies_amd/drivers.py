@@ -33,7 +33,7 @@ class LinearSolveContext:
     """State shared by the assemble-and-solve callbacks (PC set-up policy, records)."""
 
     def __init__(self, assembler, *, ksp_rtol, ksp_max_it, pc_setup_on_ksp_cap=False, save_linear_timing=False,
-                 keep_pc=False):
+                 keep_pc=False, pc_refresh_its=None):
         self.asm = assembler
         self.ksp_rtol = float(ksp_rtol)
         self.ksp_max_it = int(ksp_max_it)
@@ -41,6 +41,8 @@ class LinearSolveContext:
         # keep_pc: with pc_setup_on_ksp_cap, carry an existing preconditioner over from the
         # previous solve (e.g. the previous load step) instead of rebuilding it first
         self.keep_pc = bool(keep_pc)
+        # with pc_setup_on_ksp_cap: also rebuild once a solve needed more than this many iterations
+        self.pc_refresh_its = None if pc_refresh_its is None else int(pc_refresh_its)
         self.force_pc_setup_next = not (self.keep_pc and self._has_pc())
         self.save_linear_timing = bool(save_linear_timing)
         self.records: list[dict] = []
@@ -83,7 +85,8 @@ class LinearSolveContext:
         t_solve = time.perf_counter() - t2
         its = ksp.getIterationNumber()
         self.iters.append(int(its))
-        if self.pc_setup_on_ksp_cap and its >= self.ksp_max_it:
+        if self.pc_setup_on_ksp_cap and (its >= self.ksp_max_it or (
+                self.pc_refresh_its is not None and its > self.pc_refresh_its)):
             self.force_pc_setup_next = True
         if self.save_linear_timing:
             self.records.append(
@@ -121,6 +124,7 @@ def solve_newton(
     first_step_laplacian=False,
     verbose=False,
     keep_pc=False,
+    pc_refresh_its=None,
     **newton_kwargs,
 ):
     """Run Newton (with the reference's retry schedule) from device vector ``x`` in place.
@@ -134,7 +138,7 @@ def solve_newton(
     ls = tuple(opts.pop("linesearch_interval"))
     ctx = LinearSolveContext(
         assembler, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
-        save_linear_timing=save_linear_timing, keep_pc=keep_pc,
+        save_linear_timing=save_linear_timing, keep_pc=keep_pc, pc_refresh_its=pc_refresh_its,
     )
     if first_step_laplacian and first_direction_fn is None:
         def first_direction_fn(vec, rhs, sol):

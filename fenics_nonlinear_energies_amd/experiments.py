@@ -51,8 +51,8 @@ def he_predictor(p, asm, net, mode, total_angle):
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
-                       ksp_rtol=1e-1, pc_reuse=True, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0,
-                       guard=True, newton_kwargs=None, log=print):
+                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=None, hidden=128, layers=4, epochs=60, lr=2e-3,
+                       seed=0, guard=True, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -64,7 +64,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         pred = he_predictor(p, asm, None, "extrapolate", total_angle)
         holder = {}
         recs, _ = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
-                                      ksp_rtol=ksp_rtol, pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs,
+                                      ksp_rtol=ksp_rtol, pc_setup_on_ksp_cap=pc_reuse, pc_refresh_its=pc_refresh_its,
+                                      newton_kwargs=newton_kwargs,
                                       on_step=lambda r, h: holder.update(h=h))
         hist = holder.get("h", [])
         x, t = training_pairs(pred, hist)
@@ -94,7 +95,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         t0 = time.perf_counter()
         recs, x = solve_he_load_steps(
             asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it, ksp_rtol=ksp_rtol,
-            pc_setup_on_ksp_cap=pc_reuse, newton_kwargs=newton_kwargs,
+            pc_setup_on_ksp_cap=pc_reuse, pc_refresh_its=pc_refresh_its, newton_kwargs=newton_kwargs,
             on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -106,8 +107,11 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
             "all_converged": all("onverged" in r["message"] for r in recs),
             "per_step_nit": [r["nit"] for r in recs],
             "learned_accepted": list(pred.accepted),
+            "pc_setups": int(getattr(asm.ksp._amg, "setup_count", 0)) if asm.ksp._amg is not None else 0,
+            "pc_setup_s": float(getattr(asm.ksp._amg, "setup_time", 0.0)) if asm.ksp._amg is not None else 0.0,
         }
-        log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s E={recs[-1]['energy']:.12e}")
+        log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s E={recs[-1]['energy']:.12e} "
+            f"pc_setups={out['variants'][name]['pc_setups']} ({out['variants'][name]['pc_setup_s']:.1f}s)")
         asm.cleanup()
     ref = out["variants"]["previous_fp64"]
     for v in out["variants"].values():

@@ -60,8 +60,11 @@ def perm_block_xyz(coords_all: np.ndarray, freedofs: np.ndarray, block_size: int
 
 
 def _morton_codes(coords: np.ndarray, bits: int = 20) -> np.ndarray:
+    """Z-order codes on an isotropic grid: every axis is quantised with the largest extent,
+    so an aligned block of codes is a cube in physical space.  (Scaling each axis by its own
+    extent would turn the blocks of a 40:1:1 beam into lines along its axis.)"""
     lo = coords.min(axis=0)
-    span = np.maximum(coords.max(axis=0) - lo, 1e-300)
+    span = max(float(np.max(coords.max(axis=0) - lo)), 1e-300)
     q = np.floor((coords - lo) / span * ((1 << bits) - 1) + 0.5).astype(np.uint64)
     dim = coords.shape[1]
     code = np.zeros(coords.shape[0], dtype=np.uint64)

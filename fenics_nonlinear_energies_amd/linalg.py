@@ -138,6 +138,7 @@ class KSP:
         self.pc_setup_count = 0
         self.check_every = 16  # device CG: iterations enqueued between polls of the stop flag
         self.near_nullspace = None  # (n, k) host array for pc_type "gamg" (rigid modes, constants)
+        self.near_nullspace_fn = None  # optional callable -> (n, k) array at the current state, or None
         self._amg = None
 
     # PETSc-style configuration -----------------------------------------------------------
@@ -174,9 +175,10 @@ class KSP:
         elif self.pc_type in ("gamg", "hypre"):
             from .amg import SmoothedAggregationPC
 
-            if self._amg is None:
+            if self._amg is None or self._amg.precision != self.A.precision:
                 self._amg = SmoothedAggregationPC(self.near_nullspace, precision=self.A.precision)
-            self._amg.setup(self.A.to_scipy(), self.A.indices.device)
+            B = self.near_nullspace_fn() if self.near_nullspace_fn is not None else None
+            self._amg.setup(self.A, near_nullspace=B)
             self._inv_diag = None
         else:
             raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | gamg | none)")

@@ -102,6 +102,12 @@ _SIGNATURES = [
         C.c_int,
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
     ),
+    (
+        "fne_spgemm_numeric",
+        C.c_int,
+        [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_void_p],
+    ),
     ("fne_dot", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     (
         "fne_color_patch_elements",

@@ -155,8 +155,26 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
         # element-major element matrices + CSR row gather: measured faster than the node-block
         # patch schedule on the 160x8x8 (2.8x) and 320x16x16 (1.2x) beams
         kw.setdefault("assembly_mode", "rowgather")
+        near_nullspace_mode = kw.pop("near_nullspace_mode", "current")
         super().__init__(params, comm, adjacency, ksp_type=ksp_type, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
                          reorder_mode=reorder_mode, **kw)
+        # The unknown is the deformed position, so the rigid-body modes of the current
+        # configuration are the near-kernel of the current tangent; "reference" keeps the
+        # modes of the undeformed mesh (params["elastic_kernel"]) as the reference's GAMG does.
+        node_of = np.asarray(self.params["freedofs"], dtype=np.int64)[self.layout.perm] // 3
+        node_major = node_of.size % 3 == 0 and np.array_equal(node_of[0::3], node_of[1::3]) and np.array_equal(
+            node_of[0::3], node_of[2::3])
+        if near_nullspace_mode not in ("current", "reference"):
+            raise ValueError("near_nullspace_mode must be 'current' or 'reference'")
+        if near_nullspace_mode == "current" and node_major and getattr(self, "ksp", None) is not None:
+            self.ksp.near_nullspace_fn = self._rigid_modes_current
+
+    def _rigid_modes_current(self) -> np.ndarray | None:
+        """Rigid-body modes at the state of the last Hessian assembly (free nodes, reordered)."""
+        x = getattr(self, "_hessian_state", None)
+        if x is None:
+            return None
+        return rigid_modes(x.detach().double().cpu().numpy().reshape(-1, 3))
 
     def _element_geometry(self) -> np.ndarray:
         p = self.params
@@ -200,7 +218,8 @@ HE_NEWTON = dict(tolf=1e-4, tolg=1e-3, tolg_rel=1e-3, maxit=100, linesearch_inte
 
 def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * np.pi, *, num_steps=None,
                         predictor=None, on_step=None, ksp_rtol: float = 1e-1, ksp_max_it: int = 30,
-                        pc_setup_on_ksp_cap: bool = False, newton_kwargs: dict | None = None):
+                        pc_setup_on_ksp_cap: bool = False, pc_refresh_its: int | None = None,
+                        newton_kwargs: dict | None = None):
     """Rotate the right face in ``total_steps`` equal steps and solve each step with Newton.
 
     Follows the reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-359``):
@@ -209,7 +228,10 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
     ``predictor(history, angle)`` may return another initial guess (device tensor of free
     values); ``history`` holds (angle, free-state tensor) of the converged steps, starting with
     the reference configuration at angle 0.  The reference rotates by 4 * 2 pi in total
-    (``solver.py:225``); the angle is a parameter here.
+    (``solver.py:225``); the angle is a parameter here.  Preconditioner policy: set up at every
+    Newton iteration (the reference's), or with ``pc_setup_on_ksp_cap`` kept across iterations
+    and steps and rebuilt when a solve hits ``ksp_max_it`` or needs more than ``pc_refresh_its``
+    iterations.
     """
     import time
 
@@ -235,7 +257,8 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                           pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap, **opts)
+                           pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap,
+                           pc_refresh_its=pc_refresh_its, **opts)
         torch.cuda.synchronize()
         rec = {
             "step": k,

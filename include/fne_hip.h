@@ -171,6 +171,20 @@ int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, 
                  int32_t precision, double* diag, fne_stream_t stream);
 
 /*
+ * Numeric sparse product C = A B into C's fixed CSR pattern (c_indptr/c_indices, sorted
+ * columns; it must contain every column the row's products reach).  A values FP64 or FP32
+ * (a_precision), B and C FP64; A's column indices must be < B's row count.  Deterministic:
+ * each entry sums its products in the order of A's row.  Used by the multigrid set-up for
+ * the smoothed prolongator and the Galerkin products P^T A P, which PETSc GAMG recomputes at
+ * every PCSetUp (pc_type "gamg", hyperelasticity/jax_petsc/solver.py:138-143; the PC is set
+ * up per Newton iteration through ksp.setOperators/solve in core/petsc/scalar_problem_driver.py).
+ */
+int fne_spgemm_numeric(int32_t n_rows, const int32_t* a_indptr, const int32_t* a_indices, const void* a_vals,
+                       int32_t a_precision, const int32_t* b_indptr, const int32_t* b_indices,
+                       const double* b_vals, const int32_t* c_indptr, const int32_t* c_indices, double* c_vals,
+                       fne_stream_t stream);
+
+/*
  * Preconditioned CG on the device (KSPCG semantics: zero initial guess, point-Jacobi left
  * preconditioning via inv_diag (NULL = none), convergence when the preconditioned residual
  * norm <= max(rtol*initial, atol), divergence at dtol*initial, indefinite checks).  Replaces

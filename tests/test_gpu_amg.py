@@ -73,3 +73,148 @@ def test_vcycle_graph_replays_match_eager(require_gpu):
         assert torch.equal(z, eager), k
         assert torch.isfinite(junk).all()
     asm.cleanup()
+
+
+@pytest.mark.parametrize("a_prec", ["fp64", "fp32"])
+def test_spgemm_numeric_matches_scipy(require_gpu, a_prec):
+    """fne_spgemm_numeric against scipy's product, with result rows longer than one LDS chunk
+    (1024 entries) and signed values (cancelled entries of the fixed pattern come out 0)."""
+    import scipy.sparse as sp
+    import torch
+
+    from fenics_nonlinear_energies_amd import amg
+
+    A = sp.random(70, 300, density=0.08, random_state=3, format="csr")
+    B = sp.random(300, 4000, density=0.03, random_state=4, format="csr")
+    A.data -= 0.5
+    B.data -= 0.5
+    if a_prec == "fp32":
+        A.data = A.data.astype(np.float32).astype(np.float64)
+    C_pat = amg._pattern_product(A, B)
+    assert np.diff(C_pat.indptr).max() > 1024
+    dev = torch.device("cuda")
+    Bd = amg._DevMat(B.indptr, B.indices, B.shape[1], dev, "fp64")
+    Bd.data.copy_(torch.as_tensor(B.data))
+    Cd = amg._DevMat(C_pat.indptr, C_pat.indices, C_pat.shape[1], dev, "fp64")
+    a_ptr = torch.as_tensor(A.indptr.astype(np.int32)).to(dev)
+    a_idx = torch.as_tensor(A.indices.astype(np.int32)).to(dev)
+    a_dat = torch.as_tensor(A.data).to(dev, torch.float32 if a_prec == "fp32" else torch.float64)
+    Cd.data.fill_(np.nan)
+    amg._spgemm(a_ptr, a_idx, a_dat, Bd, Bd.data, Cd, Cd.data)
+    got = sp.csr_matrix((Cd.data.cpu().numpy(), C_pat.indices, C_pat.indptr), shape=C_pat.shape).toarray()
+    ref = (A @ B).toarray()
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+    # deterministic: a second product is bitwise the same
+    first = Cd.data.clone()
+    amg._spgemm(a_ptr, a_idx, a_dat, Bd, Bd.data, Cd, Cd.data)
+    assert torch.equal(first, Cd.data)
+
+
+def _host_levels(pc, A0):
+    """Operators of every level read back from the device (FP64 masters)."""
+    import scipy.sparse as sp
+
+    out = []
+    A = A0
+    for L in pc.levels:
+        P = L["P64"]
+        Pm = sp.csr_matrix((P.data.cpu().numpy(), P.indices.cpu().numpy(), P.indptr.cpu().numpy()),
+                           shape=(L["n"], L["nc"]))
+        Ac = L["Ac64"]
+        Acm = sp.csr_matrix((Ac.data.cpu().numpy(), Ac.indices.cpu().numpy(), Ac.indptr.cpu().numpy()),
+                            shape=(L["nc"], L["nc"]))
+        out.append((A, Pm, Acm, L["wd"].cpu().numpy()))
+        A = Acm
+    return out
+
+
+def test_device_galerkin_setup_matches_host_products(require_gpu):
+    """Every level of the device set-up: P = P_tent - omega D^-1 A P_tent with P_tent from the
+    host QR of the (current-configuration) rigid modes, A_c = P^T A P, omega within 15 % of the
+    host power-iteration value of 4 / (3 rho(D^-1 A)); and the V-cycle equals a host cycle
+    on the same operators."""
+    import torch
+
+    from fenics_nonlinear_energies_amd import amg
+
+    p, adj, u0 = build_beam_problem(40, 4, 4)
+    asm = HEReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+    u = u0 + 1e-4 * np.random.default_rng(0).standard_normal(u0.size)
+    x = asm.create_vec(u[asm.layout.perm])
+    asm.assemble_hessian(x)
+    B = asm._rigid_modes_current()
+    pc = amg.SmoothedAggregationPC(B, precision="fp64", max_coarse=100)
+    pc.setup(asm.A)
+    assert len(pc.levels) >= 2
+    Bl = B
+    for lvl, (A, P, Ac, wd) in enumerate(_host_levels(pc, asm.A.to_scipy())):
+        Pt, Bc = amg.tentative_prolongator(Bl, pc.levels[lvl]["agg"])
+        d = A.diagonal()
+        omega = wd[0] * d[0]
+        np.testing.assert_allclose(wd, omega / d, rtol=1e-12)
+        amg_rho = None  # host estimate of rho(D^-1 A)
+        v = np.random.default_rng(1).standard_normal(A.shape[0])
+        for _ in range(60):
+            y = (A @ v) / d
+            amg_rho = np.linalg.norm(y) / np.linalg.norm(v)
+            v = y / np.linalg.norm(y)
+        rho = amg_rho
+        # omega from a 12-step device power iteration: near 4 / (3 rho) and a convergent smoother
+        assert 0.85 <= omega * 3.0 * rho / 4.0 <= 1.15 and omega * rho < 2.0, (omega, rho)
+        Pref = (Pt - omega * (A @ Pt).multiply(1.0 / d[:, None])).toarray()
+        np.testing.assert_allclose(P.toarray(), Pref, rtol=0, atol=1e-12 * np.abs(Pref).max())
+        Acref = (P.T @ A @ P).toarray()
+        Acref = 0.5 * (Acref + Acref.T)
+        np.testing.assert_allclose(Ac.toarray(), Acref, rtol=0, atol=1e-12 * np.abs(Acref).max())
+        Bl = Bc
+    # V-cycle against the host restatement on the read-back operators
+    levels = _host_levels(pc, asm.A.to_scipy())
+    Ainv = pc.coarse_op.data.cpu().numpy().reshape(pc._coarse_n, pc._coarse_n)
+
+    def cycle(lvl, r):
+        if lvl == len(levels):
+            return Ainv @ r
+        A, P, _, wd = levels[lvl]
+        xx = wd * r
+        for _ in range(pc.sweeps - 1):
+            xx = xx + wd * (r - A @ xx)
+        xx = xx + P @ cycle(lvl + 1, P.T @ (r - A @ xx))
+        for _ in range(pc.sweeps):
+            xx = xx + wd * (r - A @ xx)
+        return xx
+
+    r = np.random.default_rng(2).standard_normal(asm.layout.n_free)
+    z = torch.empty(r.size, dtype=torch.float64, device="cuda")
+    pc.apply(torch.as_tensor(r).cuda(), z)
+    ref = cycle(0, r)
+    np.testing.assert_allclose(z.cpu().numpy(), ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+    asm.cleanup()
+
+
+def test_resetup_reuses_graph_and_follows_the_hessian(require_gpu):
+    """A second set-up at another state rewrites the recorded cycle's buffers in place: the
+    graph object is kept and its replays equal the eager cycle on the new operators."""
+    import torch
+
+    p, adj, u0 = build_beam_problem(40, 4, 4)
+    asm = HEReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+    ksp = asm.ksp
+    rng = np.random.default_rng(3)
+    graph = None
+    for k, scale in enumerate((1e-5, 3e-3)):
+        x = asm.create_vec((u0 + scale * rng.standard_normal(u0.size))[asm.layout.perm])
+        asm.assemble_hessian(x)
+        ksp.setOperators(asm.A)
+        ksp.setUp()
+        pc = ksp._amg
+        if graph is None:
+            graph = pc._graph
+        assert pc._graph is graph and pc.setup_count == k + 1
+        r = torch.as_tensor(rng.standard_normal(asm.layout.n_free)).cuda()
+        eager = pc._cycle(0, r.clone())
+        z = torch.empty_like(r)
+        pc.apply(r, z)
+        torch.cuda.synchronize()
+        assert torch.equal(z, eager), k
+    asm.cleanup()
