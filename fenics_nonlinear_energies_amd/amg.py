@@ -27,9 +27,10 @@ operator's precision (FP32 in the mixed-precision mode) for the cycle.
 
 Apply (device): one symmetric V-cycle -- two damped-Jacobi sweeps before and after the
 coarse correction -- with every operator product, the coarsest dense inverse included, on
-the CSR SpMV kernel (``fne_spmv``); the finest level uses the live Hessian.  The cycle is
-recorded as a HIP graph after the first set-up and replayed; later set-ups rewrite its
-buffers in place.  It is a fixed symmetric positive-definite linear operator between
+the CSR SpMV kernel (``fne_spmv``); every level, the finest included, uses the operator copied
+at set-up, so a kept preconditioner stays the same linear operator while the Hessian changes.
+The cycle is recorded as a HIP graph after the first set-up and replayed; later set-ups
+rewrite its buffers in place.  It is a fixed symmetric positive-definite linear operator between
 set-ups, so it preconditions CG.
 """
 
@@ -65,6 +66,18 @@ class _DevMat:
         self.data = torch.zeros(self.nnz, dtype=dt, device=device)
         self.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
         self.lpr = _lanes_for(self.nnz / max(1, self.n_rows))
+
+    @classmethod
+    def sharing_pattern(cls, indptr: torch.Tensor, indices: torch.Tensor, n_cols: int, precision: str) -> "_DevMat":
+        """Operator with its own values on another matrix's (device) pattern."""
+        m = cls.__new__(cls)
+        m.n_rows, m.n_cols, m.nnz = int(indptr.numel() - 1), int(n_cols), int(indices.numel())
+        m.indptr, m.indices = indptr, indices
+        dt = torch.float32 if precision == "fp32" else torch.float64
+        m.data = torch.zeros(m.nnz, dtype=dt, device=indices.device)
+        m.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
+        m.lpr = _lanes_for(m.nnz / max(1, m.n_rows))
+        return m
 
     def mult(self, x: torch.Tensor, y: torch.Tensor) -> None:
         native.check(
@@ -153,7 +166,8 @@ def tentative_prolongator(B: np.ndarray, agg_dofs: int):
 class SmoothedAggregationPC:
     """``pc_type="gamg"``: smoothed-aggregation AMG V-cycle on the device."""
 
-    POWER_ITS = 12
+    POWER_ITS = 12  # power iterations for rho(D^-1 A) at the first set-up
+    POWER_ITS_WARM = 4  # at later set-ups, started from the previous vector
 
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
                  max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
@@ -229,11 +243,11 @@ class SmoothedAggregationPC:
                 "v0": torch.as_tensor(np.random.default_rng(lvl).standard_normal(nl)).to(device),
             })
             A_pat = Ac_pat
-        # the cycle's copy of each coarse operator, in the operator's precision
-        for i in range(1, len(levels)):
-            src = levels[i - 1]["Ac64"]
-            levels[i]["A"] = _DevMat(src.indptr.cpu().numpy(), src.indices.cpu().numpy(), levels[i]["n"], device,
-                                     prec)
+        # the cycle's copy of every level's operator, in the operator's precision: the cycle
+        # stays the fixed operator of its set-up while the assembler rewrites the Hessian
+        for i, L in enumerate(levels):
+            ip, ix = (A0.indptr, A0.indices) if i == 0 else (levels[i - 1]["Ac64"].indptr, levels[i - 1]["Ac64"].indices)
+            L["A"] = _DevMat.sharing_pattern(ip, ix, L["n"], prec)
         nc = A_pat.shape[0]
         self._coarse_pat = A_pat
         cols = np.tile(np.arange(nc), nc)
@@ -256,7 +270,7 @@ class SmoothedAggregationPC:
 
     @staticmethod
     def _key_of(A0, k: int) -> tuple:
-        return (id(A0), A0.nnz, A0.indices.data_ptr(), A0.data.data_ptr(), A0.data.dtype, int(k))
+        return (id(A0), A0.nnz, A0.indices.data_ptr(), int(k))
 
     def setup(self, A0, device=None, near_nullspace: np.ndarray | None = None) -> None:
         """(Re)build the hierarchy from the device Hessian ``A0`` (a ``DeviceCSR``).
@@ -280,15 +294,18 @@ class SmoothedAggregationPC:
         self._A0 = A0
         pt_vals = self._tentative_values(B)
         a_ptr, a_idx, a_dat = A0.indptr, A0.indices, A0.data
+        if self.levels:
+            self.levels[0]["A"].data.copy_(A0.data)
         for lvl, L in enumerate(self.levels):
             d = A0.diagonal() if lvl == 0 else a_dat[L["diag"]]
             d = torch.where(d == 0.0, torch.ones_like(d), d)
             dinv = 1.0 / d
-            # rho(D^-1 A) by power iteration on the device (no host synchronisation)
+            # rho(D^-1 A) by power iteration on the device (no host synchronisation), warm-started
+            # from the previous set-up's vector
             v = L["v0"] / torch.linalg.vector_norm(L["v0"])
             y = torch.empty_like(v)
             lam = None
-            for _ in range(self.POWER_ITS):
+            for _ in range(self.POWER_ITS_WARM if L.get("warm") else self.POWER_ITS):
                 if lvl == 0:
                     A0.mult_tensor(v, y)
                 else:
@@ -296,6 +313,8 @@ class SmoothedAggregationPC:
                 y.mul_(dinv)
                 lam = torch.linalg.vector_norm(y)
                 v = y / lam
+            L["v0"].copy_(v)
+            L["warm"] = True
             omega = 4.0 / (3.0 * 1.05 * lam)
             L["wd"].copy_(omega * dinv)
             # P = P_tent - omega D^-1 (A P_tent)
@@ -319,11 +338,12 @@ class SmoothedAggregationPC:
         nc, cp = self._coarse_n, self._coarse_pat
         Ad = sp.csr_matrix((a_dat.double().cpu().numpy(), cp.indices, cp.indptr), shape=(nc, nc)).toarray()
         Ad = 0.5 * (Ad + Ad.T)
-        try:
-            Lc = sla.cholesky(Ad, lower=True)
-            Linv = sla.solve_triangular(Lc, np.eye(nc), lower=True)
-            Ainv = Linv.T @ Linv
-        except np.linalg.LinAlgError:
+        c, info = sla.lapack.dpotrf(Ad, lower=1)
+        if info == 0:
+            Ainv, info = sla.lapack.dpotri(c, lower=1)  # lower triangle of the inverse
+        if info == 0:
+            Ainv = np.tril(Ainv) + np.tril(Ainv, -1).T
+        else:
             shift = 1e-12 * np.trace(Ad) / max(1, nc)
             w, V = np.linalg.eigh(Ad)
             w = np.where(w > shift, w, np.inf)
@@ -357,10 +377,7 @@ class SmoothedAggregationPC:
         return y
 
     def _apply_A(self, lvl: int, x: torch.Tensor, y: torch.Tensor) -> None:
-        if lvl == 0:
-            self._A0.mult_tensor(x, y)
-        else:
-            self.levels[lvl]["A"].mult(x, y)
+        self.levels[lvl]["A"].mult(x, y)
 
     def _cycle(self, lvl: int, r: torch.Tensor) -> torch.Tensor:
         if lvl == len(self.levels):
@@ -384,13 +401,13 @@ class SmoothedAggregationPC:
             x.add_(wd * (r - t))
         return x
 
-    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+    def check_live(self) -> None:
+        """Raise unless set up (the cycle reads only buffers the hierarchy owns or shares)."""
         if self._A0 is None:
             raise RuntimeError("multigrid apply before setup()")
-        if self._key_of(self._A0, self._key[-1]) != self._key:
-            # the Hessian buffers moved (e.g. a precision change): the recorded cycle would
-            # read freed memory
-            raise RuntimeError("the Hessian was reallocated since the multigrid set-up; call setup() again")
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        self.check_live()
         if self._graph is not None:
             self._g_in.copy_(r)
             self._graph.replay()

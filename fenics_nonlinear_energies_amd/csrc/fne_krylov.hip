@@ -217,6 +217,40 @@ __global__ __launch_bounds__(1024) void cg_beta_finalize(CgParams P, const doubl
   st[kRatio] = beta / beta_old;
 }
 
+// graph-preconditioned variant: x += alpha p, r -= alpha w (z comes from the preconditioner)
+__global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int n, const double* __restrict__ p,
+                                                              const double* __restrict__ w, double* __restrict__ x,
+                                                              double* __restrict__ r, const double* __restrict__ st) {
+  if (st[kDone] != 0.0) return;
+  const double a = st[kAlpha];
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x[i] += a * p[i];
+    r[i] -= a * w[i];
+  }
+}
+
+// partials of z.z and z.r (z = M r from the preconditioner)
+__global__ __launch_bounds__(kBlock) void cg_dots_kernel(int n, const double* __restrict__ z,
+                                                         const double* __restrict__ r, const double* __restrict__ st,
+                                                         double2* __restrict__ part) {
+  if (st != nullptr && st[kDone] != 0.0) return;
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double zi = z[i];
+    acc_add(s1, c1, zi * zi);
+    acc_add(s2, c2, zi * r[i]);
+  }
+  block_reduce_pair(s1, c1);
+  __syncthreads();
+  block_reduce_pair(s2, c2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = make_double2(s1, c1);
+    part[gridDim.x + blockIdx.x] = make_double2(s2, c2);
+  }
+}
+
 constexpr int kVecBlocks = 1024;
 
 int lanes_for(int n, int64_t nnz) {
@@ -329,3 +363,83 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
 }
 
 }  // extern "C"
+
+extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
+                             int32_t precision, void* pc_graph_exec, double* pc_in, const double* pc_out,
+                             const double* b, double* x, double rtol, double atol, double dtol, int32_t max_it,
+                             int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+                             int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  if (n < 0 || !indptr || !indices || !vals || !b || !x || !work || max_it < 0 || !pc_graph_exec || !pc_in ||
+      !pc_out || pc_in == pc_out) {
+    fne_set_error("fne_pcg_graph: invalid arguments");
+    return FNE_EINVAL;
+  }
+  if (precision != FNE_FP32 && precision != FNE_FP64) {
+    fne_set_error("fne_pcg_graph: precision must be FNE_FP64 or FNE_FP32");
+    return FNE_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipGraphExec_t pc = (hipGraphExec_t)pc_graph_exec;
+  int64_t nnz64 = 0;
+  {
+    int32_t last = 0;
+    if (n > 0 && hipMemcpyAsync(&last, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) {
+      fne_set_error("fne_pcg_graph: reading indptr failed");
+      return FNE_EHIP;
+    }
+    FNE_HIP_CHECK(hipStreamSynchronize(s));
+    nnz64 = last;
+  }
+  if (work_bytes < fne_pcg_workspace_bytes(n, nnz64)) {
+    fne_set_error("fne_pcg_graph: workspace too small");
+    return FNE_EINVAL;
+  }
+  if (check_every < 1) check_every = 1;
+  const int lpr = lanes_for(n, nnz64);
+  const int nbs = int(spmv_blocks(n, lpr));
+  // r lives in the preconditioner's input buffer, z in its output buffer
+  double* r = pc_in;
+  const double* z = pc_out;
+  double* scratch_z = reinterpret_cast<double*>(work);  // written by the shared init kernel, unused
+  double* w = scratch_z + n;
+  double* pa = w + n;
+  double* pb = pa + n;
+  double* st = pb + 2 * int64_t(n);
+  double2* part = reinterpret_cast<double2*>(st + kStateLen);
+  CgParams P{n, indptr, indices, nullptr, rtol, atol, dtol};
+  const int nbv = n > 0 ? int(std::min<int64_t>(kVecBlocks, (int64_t(n) + kBlock - 1) / kBlock)) : 1;
+  hipLaunchKernelGGL(cg_init_kernel, dim3(nbv), dim3(kBlock), 0, s, P, b, x, r, scratch_z, pa, part);
+  FNE_HIP_CHECK(hipGraphLaunch(pc, s));
+  hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, z, r, nullptr, part);
+  hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
+  FNE_LAUNCH_CHECK();
+  double hst[kStateLen];
+  int it = 0;
+  while (true) {
+    FNE_HIP_CHECK(hipMemcpyAsync(hst, st, sizeof(hst), hipMemcpyDeviceToHost, s));
+    FNE_HIP_CHECK(hipStreamSynchronize(s));
+    if (hst[kDone] != 0.0 || it >= max_it) break;
+    const int upto = std::min<int>(max_it, it + check_every);
+    for (; it < upto; ++it) {
+      double* p_old = (it & 1) ? pb : pa;
+      double* p_new = (it & 1) ? pa : pb;
+      if (precision == FNE_FP32)
+        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, st, part, nbs, s);
+      else
+        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, st, part, nbs, s);
+      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, part, nbs, st);
+      hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nbv), dim3(kBlock), 0, s, n, p_new, w, x, r, st);
+      FNE_HIP_CHECK(hipGraphLaunch(pc, s));
+      hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, z, r, st, part);
+      hipLaunchKernelGGL(cg_beta_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
+    }
+    FNE_LAUNCH_CHECK();
+  }
+  const int its = int(hst[kIts]);
+  int reason = int(hst[kReason]);
+  if (hst[kDone] == 0.0) reason = -3;  // KSP_DIVERGED_ITS
+  if (its_out) *its_out = its;
+  if (reason_out) *reason_out = reason;
+  if (rnorm_out) *rnorm_out = hst[kRnorm];
+  return FNE_OK;
+}

@@ -137,6 +137,7 @@ class KSP:
         self.rnorm = float("nan")
         self.pc_setup_count = 0
         self.check_every = 16  # device CG: iterations enqueued between polls of the stop flag
+        self.check_every_graph_pc = 1  # the same with the multigrid graph (a wasted cycle costs more)
         self.near_nullspace = None  # (n, k) host array for pc_type "gamg" (rigid modes, constants)
         self.near_nullspace_fn = None  # optional callable -> (n, k) array at the current state, or None
         self._amg = None
@@ -237,7 +238,8 @@ class KSP:
     def _solve_cg(self, b: torch.Tensor, x: torch.Tensor) -> None:
         import ctypes as C
 
-        if self._amg is not None:  # multigrid preconditioner: host-driven iteration
+        amg = self._amg
+        if amg is not None and getattr(amg, "_graph", None) is None:  # multigrid without a recorded cycle
             self._solve_cg_host(b, x)
             return
 
@@ -249,13 +251,29 @@ class KSP:
         if getattr(self, "_cg_work", None) is None or self._cg_work.numel() < nbytes:
             self._cg_work = torch.empty(nbytes, dtype=torch.uint8, device=b.device)
         its, reason, rnorm = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
-        native.check(
-            lib.fne_pcg(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data), A.fne_precision,
-                        native.ptr(self._inv_diag), native.ptr(b), native.ptr(x), self.rtol, self.atol, self.dtol,
-                        self.max_it, int(self.check_every), native.ptr(self._cg_work), nbytes, C.byref(its),
-                        C.byref(reason), C.byref(rnorm), native.stream_ptr()),
-            "fne_pcg",
-        )
+        if amg is not None:
+            # the V-cycle graph is launched from inside the device iteration (fne_pcg_graph);
+            # solves are short with a multigrid preconditioner, so the stop flag is read often
+            amg.check_live()
+            if amg._A0 is not A:
+                raise RuntimeError("the KSP operator changed since the multigrid set-up; call setUp()")
+            native.check(
+                lib.fne_pcg_graph(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data),
+                                  A.fne_precision, C.c_void_p(amg._graph.raw_cuda_graph_exec()), native.ptr(amg._g_in),
+                                  native.ptr(amg._g_out), native.ptr(b), native.ptr(x), self.rtol, self.atol,
+                                  self.dtol, self.max_it, int(self.check_every_graph_pc),
+                                  native.ptr(self._cg_work), nbytes, C.byref(its), C.byref(reason), C.byref(rnorm),
+                                  native.stream_ptr()),
+                "fne_pcg_graph",
+            )
+        else:
+            native.check(
+                lib.fne_pcg(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data), A.fne_precision,
+                            native.ptr(self._inv_diag), native.ptr(b), native.ptr(x), self.rtol, self.atol, self.dtol,
+                            self.max_it, int(self.check_every), native.ptr(self._cg_work), nbytes, C.byref(its),
+                            C.byref(reason), C.byref(rnorm), native.stream_ptr()),
+                "fne_pcg",
+            )
         self.its = int(its.value)
         self.reason = int(reason.value)
         self.rnorm = float(rnorm.value)

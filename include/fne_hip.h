@@ -200,6 +200,21 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
             int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
             int32_t* reason_out, double* rnorm_out, fne_stream_t stream);
 
+/*
+ * The same CG with a preconditioner given as a recorded HIP graph (e.g. the multigrid
+ * V-cycle, pc_type "gamg"): pc_graph_exec is a hipGraphExec_t whose launch computes
+ * pc_out = M pc_in, reading nothing else that the solve writes.  The residual lives in pc_in
+ * and the preconditioned residual in pc_out, so the solve overwrites pc_in.  Same semantics,
+ * workspace and outputs as fne_pcg; the iteration runs on the device, the host reads the
+ * stop flag every check_every iterations (iterations enqueued after the stop are no-ops
+ * except for the graph launch).  Replaces ksp.solve with ksp_type "cg" and pc_type "gamg"
+ * (hyperelasticity/jax_petsc/solver.py:138-143; core/petsc/scalar_problem_driver.py:196).
+ */
+int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+                  void* pc_graph_exec, double* pc_in, const double* pc_out, const double* b, double* x, double rtol,
+                  double atol, double dtol, int32_t max_it, int32_t check_every, void* work, int64_t work_bytes,
+                  int32_t* its_out, int32_t* reason_out, double* rnorm_out, fne_stream_t stream);
+
 /* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
 int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
 

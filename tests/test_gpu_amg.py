@@ -218,3 +218,31 @@ def test_resetup_reuses_graph_and_follows_the_hessian(require_gpu):
         torch.cuda.synchronize()
         assert torch.equal(z, eager), k
     asm.cleanup()
+
+
+def test_device_cg_with_graph_preconditioner_matches_host_cg(require_gpu):
+    """fne_pcg_graph (device iteration launching the recorded V-cycle) against the host-driven
+    KSPCG restatement with the same preconditioner: same iteration count and reason, and the
+    same solution to 1e-10."""
+    import torch
+
+    p, adj, u0 = build_beam_problem(40, 4, 4)
+    asm = HEReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg")
+    x = asm.create_vec((u0 + 1e-5 * np.random.default_rng(4).standard_normal(u0.size))[asm.layout.perm])
+    asm.assemble_hessian(x)
+    ksp = asm.ksp
+    ksp.setOperators(asm.A)
+    ksp.setUp()
+    assert ksp._amg._graph is not None
+    b = asm.create_vec(np.random.default_rng(5).standard_normal(asm.layout.n_free))
+    for rtol in (1e-1, 1e-8):
+        ksp.setTolerances(rtol=rtol, max_it=500)
+        xd = b.duplicate()
+        ksp.solve(b, xd)
+        its_d, reason_d = ksp.getIterationNumber(), ksp.getConvergedReason()
+        xh = torch.empty_like(b.t)
+        ksp._solve_cg_host(b.t, xh)
+        assert (its_d, reason_d) == (ksp.getIterationNumber(), ksp.getConvergedReason()), (its_d, reason_d)
+        assert reason_d > 0
+        torch.testing.assert_close(xd.t, xh, rtol=1e-10, atol=1e-10 * float(xh.abs().max()))
+    asm.cleanup()
