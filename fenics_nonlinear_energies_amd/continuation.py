@@ -61,6 +61,7 @@ def run_lambda_continuation(
     ksp_rtol: float = 1e-3,
     ksp_max_it: int = 2000,
     pc_setup_on_ksp_cap: bool = True,
+    pc_refresh_its: int | None = None,
     newton_kwargs: dict | None = None,
 ) -> list[dict]:
     """Solve the reduced-strength problem for each lambda, warm-starting from the history.
@@ -68,6 +69,8 @@ def run_lambda_continuation(
     ``x0``: device vector (e.g. the elastic initial guess), overwritten with the last state.
     ``predictor(history, lam)`` returns an initial guess tensor or None (then the previous
     converged state is used).  ``history`` holds (lam, state tensor) pairs of converged steps.
+    With ``pc_setup_on_ksp_cap`` the preconditioner is kept across Newton iterations and steps
+    and rebuilt when a solve hits ``ksp_max_it`` or needs more than ``pc_refresh_its``.
     """
     p = asm.params
     phi_raw = float(p.get("phi_deg_raw", p["phi_deg"]))
@@ -87,7 +90,7 @@ def run_lambda_continuation(
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
-                           keep_pc=pc_setup_on_ksp_cap, **opts)
+                           keep_pc=pc_setup_on_ksp_cap, pc_refresh_its=pc_refresh_its, **opts)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         rec = {

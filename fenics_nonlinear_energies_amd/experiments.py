@@ -51,7 +51,7 @@ def he_predictor(p, asm, net, mode, total_angle):
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
-                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=None, hidden=128, layers=4, epochs=60, lr=2e-3,
+                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=60, lr=2e-3,
                        seed=0, guard=True, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
@@ -142,8 +142,8 @@ def slope_predictor(p, asm, net, mode, lam_scale=1.0):
 
 
 def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.0, eval_h=0.5, degree=4,
-                          ksp_max_it=2000, ksp_rtol=1e-3, hidden=128, layers=4, epochs=60, lr=2e-3, seed=0,
-                          guard=True, newton_kwargs=None, log=print):
+                          ksp_max_it=2000, ksp_rtol=1e-3, pc_refresh_its=20, hidden=128, layers=4, epochs=60,
+                          lr=2e-3, seed=0, guard=True, newton_kwargs=None, log=print):
     """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (FP32)."""
     from .continuation import elastic_initial_guess, run_lambda_continuation
 
@@ -157,7 +157,8 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         pred = slope_predictor(p, asm, None, "extrapolate")
         holder = {}
         recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                                       newton_kwargs=newton_kwargs, on_step=lambda r, h: holder.update(h=h))
+                                       pc_refresh_its=pc_refresh_its, newton_kwargs=newton_kwargs,
+                                       on_step=lambda r, h: holder.update(h=h))
         xs, ts = training_pairs(pred, holder.get("h", []))
         feats.append(xs)
         targs.append(ts)
@@ -179,7 +180,7 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         if guard:
             pred.energy_fn = lambda t, a=asm: float(a.energy_device(t).item())
         recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                                       newton_kwargs=newton_kwargs)
+                                       pc_refresh_its=pc_refresh_its, newton_kwargs=newton_kwargs)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out["variants"][name] = {
@@ -191,6 +192,7 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
             "per_step_nit": [r["nit"] for r in recs],
             "learned_accepted": list(pred.accepted),
             "n_free": int(asm.layout.n_free),
+            "pc_setups": int(getattr(asm.ksp._amg, "setup_count", 0)) if asm.ksp._amg is not None else 0,
         }
         log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s")
         asm.cleanup()
