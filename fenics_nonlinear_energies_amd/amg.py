@@ -16,12 +16,13 @@ it is smoothed once with damped Jacobi, P = (I - omega D^-1 A) P_tent with
 omega = 4 / (3 rho(D^-1 A)); coarse operators are Galerkin products P^T A P, recursively,
 until the coarse problem is small enough for a dense inverse.
 
-Set-up in two phases.  Symbolic (host, scipy, once per Hessian pattern): the patterns of
-P, A P, P^T and P^T A P on every level, the transpose permutations and the diagonal
-positions.  Numeric (device, every set-up): the per-aggregate QR of the near-nullspace
-(host, batched numpy -- the near-nullspace may follow the current state), the spectral radius
-by power iteration, and the three products per level on the fixed-pattern product kernel
-(``fne_spgemm_numeric``); only the coarsest operator goes to the host for its dense inverse.
+Set-up in two phases.  Symbolic (host, scipy, once per Hessian pattern, done by the
+assembler at construction): the patterns of P, A P, P^T and P^T A P on every level, the
+transpose permutations and the diagonal positions.  Numeric (device, every set-up): the
+per-aggregate QR of the near-nullspace (``fne_aggregate_qr``; the near-nullspace may follow
+the current state), the spectral radius by power iteration, and the three products per level
+on the fixed-pattern product kernel (``fne_spgemm_numeric``); only the coarsest operator goes
+to the host for its dense inverse.
 Level operators and transfers are kept in FP64 for the products and copied to the
 operator's precision (FP32 in the mixed-precision mode) for the cycle.
 
@@ -53,6 +54,10 @@ def _lanes_for(avg: float) -> int:
     return lpr
 
 
+# fne_vcycle_op operations
+_SWEEP_FIRST, _SWEEP, _RESIDUAL, _ADD_PRODUCT = 0, 1, 2, 3
+
+
 class _DevMat:
     """CSR operator on the device (rows may differ from columns); ``data`` is rewritten in place."""
 
@@ -78,6 +83,16 @@ class _DevMat:
         m.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
         m.lpr = _lanes_for(m.nnz / max(1, m.n_rows))
         return m
+
+    def op(self, op: int, x, r, w, y_in, y: torch.Tensor) -> None:
+        """Fused V-cycle operator (fne_vcycle_op): sweep / residual / prolongation-add."""
+        native.check(
+            native.lib().fne_vcycle_op(op, self.n_rows, native.ptr(self.indptr), native.ptr(self.indices),
+                                       native.ptr(self.data), self.prec | (self.lpr << 8), native.ptr(x),
+                                       native.ptr(r), native.ptr(w), native.ptr(y_in), native.ptr(y),
+                                       native.stream_ptr()),
+            "fne_vcycle_op",
+        )
 
     def mult(self, x: torch.Tensor, y: torch.Tensor) -> None:
         native.check(
@@ -139,18 +154,25 @@ def tentative_prolongator(B: np.ndarray, agg_dofs: int):
     """Block-diagonal P_tent (n x sum_a min(rows_a, k), CSR) and the coarse near-nullspace.
 
     Aggregate a holds rows [a*agg, (a+1)*agg); its block is the Q factor of those rows of B
-    (batched numpy QR) and its coarse near-nullspace rows are the R factor.
+    (batched numpy QR, signs chosen so that R has a non-negative diagonal) and its coarse
+    near-nullspace rows are the R factor.  Host twin of ``fne_aggregate_qr``.
     """
     n, k = B.shape
     sizes = _aggregate_sizes(n, agg_dofs)
     na_full = n // agg_dofs
     vals, Bc = [], []
+    def signed(Q, R):  # R with a non-negative diagonal (the convention of fne_aggregate_qr)
+        kk = Q.shape[-1]
+        d = np.diagonal(R[..., :kk], axis1=-2, axis2=-1)
+        sg = np.where(d < 0.0, -1.0, 1.0)
+        return Q * sg[..., None, :], R * sg[..., :, None]
+
     if na_full:
-        Q, R = np.linalg.qr(B[: na_full * agg_dofs].reshape(na_full, agg_dofs, k))
+        Q, R = signed(*np.linalg.qr(B[: na_full * agg_dofs].reshape(na_full, agg_dofs, k)))
         vals.append(Q.reshape(-1))
         Bc.append(R.reshape(-1, k))
     if sizes.size > na_full:
-        Q, R = np.linalg.qr(B[na_full * agg_dofs:])
+        Q, R = signed(*np.linalg.qr(B[na_full * agg_dofs:]))
         vals.append(Q.reshape(-1))
         Bc.append(R)
     kk = np.minimum(sizes, k)
@@ -240,6 +262,7 @@ class SmoothedAggregationPC:
                 "tperm_ac": torch.as_tensor(_transpose_perm(Ac_pat)).to(device),
                 "diag": torch.as_tensor(diag).to(device),
                 "wd": torch.zeros(nl, dtype=torch.float64, device=device),
+                "Bc": torch.zeros((nc, k), dtype=torch.float64, device=device),
                 "v0": torch.as_tensor(np.random.default_rng(lvl).standard_normal(nl)).to(device),
             })
             A_pat = Ac_pat
@@ -272,6 +295,16 @@ class SmoothedAggregationPC:
     def _key_of(A0, k: int) -> tuple:
         return (id(A0), A0.nnz, A0.indices.data_ptr(), int(k))
 
+    def prepare(self, A0, k: int, device=None) -> None:
+        """Symbolic phase only (host patterns for the Hessian pattern of ``A0`` and k
+        near-nullspace vectors); ``setup`` then starts with the numeric phase."""
+        device = A0.indices.device if device is None else device
+        key = self._key_of(A0, k)
+        if key != self._key:
+            self._graph = None
+            self._symbolic(A0, device, k)
+            self._key = key
+
     def setup(self, A0, device=None, near_nullspace: np.ndarray | None = None) -> None:
         """(Re)build the hierarchy from the device Hessian ``A0`` (a ``DeviceCSR``).
 
@@ -281,18 +314,31 @@ class SmoothedAggregationPC:
         t_setup = time.perf_counter()
         device = A0.indices.device if device is None else device
         if near_nullspace is not None:
-            self.B = np.asarray(near_nullspace, dtype=np.float64)
+            self.B = near_nullspace if isinstance(near_nullspace, torch.Tensor) else np.asarray(
+                near_nullspace, dtype=np.float64)
         n = A0.n
         B = self.B if self.B is not None else np.ones((n, 1))
         if B.shape[0] != n:
             raise ValueError(f"near-nullspace has {B.shape[0]} rows for an operator of size {n}")
-        key = self._key_of(A0, B.shape[1])
-        if key != self._key:
-            self._graph = None
-            self._symbolic(A0, device, B.shape[1])
-            self._key = key
+        k = int(B.shape[1])
+        self.prepare(A0, k, device)
         self._A0 = A0
-        pt_vals = self._tentative_values(B)
+        if k in (1, 2, 3, 6):
+            # tentative prolongators on the device: QR of every aggregate, level by level
+            if isinstance(B, torch.Tensor):
+                Bl = B.to(device=device, dtype=torch.float64).contiguous()
+            else:
+                if getattr(self, "_B_host", None) is not B:
+                    self._B_host, self._B_dev = B, torch.as_tensor(np.ascontiguousarray(B)).to(device)
+                Bl = self._B_dev
+            for L in self.levels:
+                native.check(native.lib().fne_aggregate_qr(L["n"], k, L["agg"], native.ptr(Bl), native.ptr(L["Pt"].data),
+                                                           native.ptr(L["Bc"]), native.stream_ptr()), "fne_aggregate_qr")
+                Bl = L["Bc"]
+            pt_vals = None
+        else:
+            Bh = B.detach().cpu().numpy() if isinstance(B, torch.Tensor) else B
+            pt_vals = self._tentative_values(Bh)
         a_ptr, a_idx, a_dat = A0.indptr, A0.indices, A0.data
         if self.levels:
             self.levels[0]["A"].data.copy_(A0.data)
@@ -319,7 +365,8 @@ class SmoothedAggregationPC:
             L["wd"].copy_(omega * dinv)
             # P = P_tent - omega D^-1 (A P_tent)
             Pt, P64 = L["Pt"], L["P64"]
-            Pt.data.copy_(torch.as_tensor(pt_vals[lvl]))
+            if pt_vals is not None:
+                Pt.data.copy_(torch.as_tensor(pt_vals[lvl]))
             _spgemm(a_ptr, a_idx, a_dat, Pt, Pt.data, P64, P64.data)
             P64.data.mul_(-omega * dinv[L["p_rows"]])
             P64.data.index_add_(0, L["ptpos"], Pt.data)
@@ -376,29 +423,30 @@ class SmoothedAggregationPC:
         self.coarse_op.mult(r, y)
         return y
 
-    def _apply_A(self, lvl: int, x: torch.Tensor, y: torch.Tensor) -> None:
-        self.levels[lvl]["A"].mult(x, y)
-
     def _cycle(self, lvl: int, r: torch.Tensor) -> torch.Tensor:
+        """One V-cycle from level ``lvl``: a fused CSR pass per sweep, residual and prolongation
+        (x_{k+1} = x_k + w (r - A x_k), w = omega D^-1, starting from x = 0)."""
         if lvl == len(self.levels):
             return self._coarse_solve(r)
         L = self.levels[lvl]
-        wd = L["wd"]
-        x = wd * r
+        A, wd = L["A"], L["wd"]
         t = torch.empty_like(r)
-        for _ in range(self.sweeps - 1):
-            self._apply_A(lvl, x, t)
-            x.add_(wd * (r - t))
-        self._apply_A(lvl, x, t)
-        res = r - t
+        if self.sweeps == 1:
+            x = wd * r
+        else:
+            x = torch.empty_like(r)
+            A.op(_SWEEP_FIRST, None, r, wd, None, x)  # the first two sweeps from x = 0
+            for _ in range(self.sweeps - 2):
+                A.op(_SWEEP, x, r, wd, None, t)
+                x, t = t, x
+        A.op(_RESIDUAL, x, r, None, None, t)
         rc = torch.empty(L["R"].n_rows, dtype=r.dtype, device=r.device)
-        L["R"].mult(res, rc)
+        L["R"].mult(t, rc)
         xc = self._cycle(lvl + 1, rc)
-        L["P"].mult(xc, t)
-        x.add_(t)
+        L["P"].op(_ADD_PRODUCT, xc, None, None, x, x)
         for _ in range(self.sweeps):
-            self._apply_A(lvl, x, t)
-            x.add_(wd * (r - t))
+            A.op(_SWEEP, x, r, wd, None, t)
+            x, t = t, x
         return x
 
     def check_live(self) -> None:

@@ -175,6 +175,9 @@ class ReorderedElementAssemblerBase:
         self.ksp = KSP(ksp_type=ksp_type, pc_type=pc_type, rtol=ksp_rtol, max_it=ksp_max_it)
         self.ksp.near_nullspace = self._near_nullspace_reordered()
         self._setup_timings["matrix_create"] = time.perf_counter() - t0
+        # multigrid symbolic phase (patterns of the Galerkin products): mesh preprocessing,
+        # done once here like the layout and the assembly plans
+        self._setup_timings["pc_symbolic"] = self.ksp.prepare_pc(self.A)
         self._energy_out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._setup_timings["total"] = time.perf_counter() - t_total
 

@@ -177,13 +177,31 @@ class KSP:
             from .amg import SmoothedAggregationPC
 
             if self._amg is None or self._amg.precision != self.A.precision:
-                self._amg = SmoothedAggregationPC(self.near_nullspace, precision=self.A.precision)
+                prepared = getattr(self, "_amg_prepared", None)
+                if prepared is not None and prepared.precision == self.A.precision:
+                    self._amg = prepared
+                else:
+                    self._amg = SmoothedAggregationPC(self.near_nullspace, precision=self.A.precision)
             B = self.near_nullspace_fn() if self.near_nullspace_fn is not None else None
             self._amg.setup(self.A, near_nullspace=B)
             self._inv_diag = None
         else:
             raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | gamg | none)")
         self.pc_setup_count += 1
+
+    def prepare_pc(self, A: DeviceCSR) -> float:
+        """Mesh preprocessing for pc_type "gamg": the multigrid's symbolic phase (patterns of
+        every level's products) for the operator pattern of ``A``.  Returns its host time; the
+        first setUp then only runs the numeric phase."""
+        if self.pc_type not in ("gamg", "hypre"):
+            return 0.0
+        from .amg import SmoothedAggregationPC
+
+        pc = SmoothedAggregationPC(self.near_nullspace, precision=A.precision)
+        k = 1 if self.near_nullspace is None else int(np.asarray(self.near_nullspace).shape[1])
+        pc.prepare(A, k)
+        self._amg_prepared = pc
+        return pc.symbolic_time
 
     def getIterationNumber(self) -> int:
         return int(self.its)

@@ -169,12 +169,22 @@ class HEReorderedElementAssembler(ReorderedElementAssemblerBase):
         if near_nullspace_mode == "current" and node_major and getattr(self, "ksp", None) is not None:
             self.ksp.near_nullspace_fn = self._rigid_modes_current
 
-    def _rigid_modes_current(self) -> np.ndarray | None:
-        """Rigid-body modes at the state of the last Hessian assembly (free nodes, reordered)."""
+    def _rigid_modes_current(self):
+        """Rigid-body modes at the state of the last Hessian assembly (free nodes, reordered),
+        as an (n, 6) device tensor laid out like ``rigid_modes``."""
+        import torch
+
         x = getattr(self, "_hessian_state", None)
         if x is None:
             return None
-        return rigid_modes(x.detach().double().cpu().numpy().reshape(-1, 3))
+        X = x.detach().to(torch.float64).reshape(-1, 3)
+        B = torch.zeros((x.numel(), 6), dtype=torch.float64, device=x.device)
+        for c in range(3):
+            B[c::3, c] = 1.0
+        B[1::3, 3], B[2::3, 3] = -X[:, 2], X[:, 1]
+        B[0::3, 4], B[2::3, 4] = X[:, 2], -X[:, 0]
+        B[0::3, 5], B[1::3, 5] = -X[:, 1], X[:, 0]
+        return B
 
     def _element_geometry(self) -> np.ndarray:
         p = self.params

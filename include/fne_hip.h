@@ -179,6 +179,32 @@ int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, 
  * every PCSetUp (pc_type "gamg", hyperelasticity/jax_petsc/solver.py:138-143; the PC is set
  * up per Newton iteration through ksp.setOperators/solve in core/petsc/scalar_problem_driver.py).
  */
+/*
+ * Tentative prolongator of the multigrid set-up: thin QR of every aggregate's rows of the
+ * near-nullspace B (n x k row-major FP64, k in {1, 2, 3, 6}); aggregate a holds rows
+ * [a*agg, min(n, (a+1)*agg)), k <= agg <= 256.  q_vals receives the Q blocks in CSR order
+ * (aggregate a's m x min(m, k) block, row-major, at offset a*agg*k), bc the R factors (the
+ * coarse near-nullspace, min(m, k) x k row-major at offset a*k*k).  R has a non-negative
+ * diagonal.  Stands in for the near-nullspace interpolation of PETSc GAMG
+ * (MatSetNearNullSpace + pc_type "gamg", hyperelasticity/jax_petsc/solver.py:138-217).
+ */
+int fne_aggregate_qr(int32_t n, int32_t k, int32_t agg, const double* B, double* q_vals, double* bc,
+                     fne_stream_t stream);
+
+/*
+ * Fused multigrid operator, one CSR pass (precision: FNE_FP64/FNE_FP32 | lanes_per_row << 8):
+ *   op 0: y = w.r + w.(r - A (w.r))   first damped-Jacobi sweep from zero (x unused)
+ *   op 1: y = x + w.(r - A x)          damped-Jacobi sweep
+ *   op 2: y = r - A x                  residual
+ *   op 3: y = y_in + A x               prolongation added to the iterate (y may equal y_in)
+ * x, r, w, y_in, y: FP64 device vectors; y must not alias x, r or w.  The smoother, residual
+ * and prolongation of the multigrid V-cycle (stands in for the level smoothers of PETSc
+ * GAMG, pc_type "gamg", hyperelasticity/jax_petsc/solver.py:138-143).
+ */
+int fne_vcycle_op(int32_t op, int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
+                  int32_t precision, const double* x, const double* r, const double* w, const double* y_in,
+                  double* y, fne_stream_t stream);
+
 int fne_spgemm_numeric(int32_t n_rows, const int32_t* a_indptr, const int32_t* a_indices, const void* a_vals,
                        int32_t a_precision, const int32_t* b_indptr, const int32_t* b_indices,
                        const double* b_vals, const int32_t* c_indptr, const int32_t* c_indices, double* c_vals,

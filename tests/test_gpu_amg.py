@@ -9,7 +9,11 @@ import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
 
-from fenics_nonlinear_energies_amd.problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
+from fenics_nonlinear_energies_amd.problems.hyperelasticity import (
+    HEReorderedElementAssembler,
+    build_beam_problem,
+    rigid_modes,
+)
 from fenics_nonlinear_energies_amd.problems.plaplace import PLaplaceReorderedElementAssembler, build_plaplace_problem
 
 pytestmark = pytest.mark.gpu
@@ -78,20 +82,21 @@ def test_vcycle_graph_replays_match_eager(require_gpu):
 @pytest.mark.parametrize("a_prec", ["fp64", "fp32"])
 def test_spgemm_numeric_matches_scipy(require_gpu, a_prec):
     """fne_spgemm_numeric against scipy's product, with result rows longer than one LDS chunk
-    (1024 entries) and signed values (cancelled entries of the fixed pattern come out 0)."""
+    (512 entries) and A rows longer than one staging chunk (256), with signed values (cancelled entries of the fixed pattern come out 0)."""
     import scipy.sparse as sp
     import torch
 
     from fenics_nonlinear_energies_amd import amg
 
-    A = sp.random(70, 300, density=0.08, random_state=3, format="csr")
-    B = sp.random(300, 4000, density=0.03, random_state=4, format="csr")
+    A = sp.random(70, 3000, density=0.1, random_state=3, format="csr")
+    B = sp.random(3000, 4000, density=0.003, random_state=4, format="csr")
     A.data -= 0.5
     B.data -= 0.5
     if a_prec == "fp32":
         A.data = A.data.astype(np.float32).astype(np.float64)
     C_pat = amg._pattern_product(A, B)
-    assert np.diff(C_pat.indptr).max() > 1024
+    assert np.diff(C_pat.indptr).max() > 1024  # more than two LDS chunks of result columns
+    assert np.diff(A.indptr).max() > 256  # more than one staging chunk of A's row
     dev = torch.device("cuda")
     Bd = amg._DevMat(B.indptr, B.indices, B.shape[1], dev, "fp64")
     Bd.data.copy_(torch.as_tensor(B.data))
@@ -143,7 +148,8 @@ def test_device_galerkin_setup_matches_host_products(require_gpu):
     u = u0 + 1e-4 * np.random.default_rng(0).standard_normal(u0.size)
     x = asm.create_vec(u[asm.layout.perm])
     asm.assemble_hessian(x)
-    B = asm._rigid_modes_current()
+    B = asm._rigid_modes_current().cpu().numpy()
+    np.testing.assert_array_equal(B, rigid_modes(x.t.cpu().numpy().reshape(-1, 3)))
     pc = amg.SmoothedAggregationPC(B, precision="fp64", max_coarse=100)
     pc.setup(asm.A)
     assert len(pc.levels) >= 2
@@ -162,6 +168,9 @@ def test_device_galerkin_setup_matches_host_products(require_gpu):
         rho = amg_rho
         # omega from a 12-step device power iteration: near 4 / (3 rho) and a convergent smoother
         assert 0.85 <= omega * 3.0 * rho / 4.0 <= 1.15 and omega * rho < 2.0, (omega, rho)
+        Pdev = pc.levels[lvl]["Pt"].data.cpu().numpy()  # device QR (fne_aggregate_qr)
+        np.testing.assert_allclose(Pdev, Pt.data, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(pc.levels[lvl]["Bc"].cpu().numpy(), Bc, rtol=0, atol=1e-12 * np.abs(Bc).max())
         Pref = (Pt - omega * (A @ Pt).multiply(1.0 / d[:, None])).toarray()
         np.testing.assert_allclose(P.toarray(), Pref, rtol=0, atol=1e-12 * np.abs(Pref).max())
         Acref = (P.T @ A @ P).toarray()
@@ -246,3 +255,46 @@ def test_device_cg_with_graph_preconditioner_matches_host_cg(require_gpu):
         assert reason_d > 0
         torch.testing.assert_close(xd.t, xh, rtol=1e-10, atol=1e-10 * float(xh.abs().max()))
     asm.cleanup()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_vcycle_ops_match_numpy(require_gpu, prec):
+    """fne_vcycle_op: first sweep, sweep, residual and in-place prolongation-add against numpy
+    on a random CSR operator (FP32 values are compared with their FP32-rounded numpy twin)."""
+    import scipy.sparse as sp
+    import torch
+
+    from fenics_nonlinear_energies_amd import amg
+
+    rng = np.random.default_rng(7)
+    A = sp.random(500, 500, density=0.03, random_state=8, format="csr") + sp.eye(500, format="csr")
+    A.sort_indices()
+    P = sp.random(500, 80, density=0.05, random_state=9, format="csr")
+    P.sort_indices()
+    if prec == "fp32":
+        A.data = A.data.astype(np.float32).astype(np.float64)
+        P.data = P.data.astype(np.float32).astype(np.float64)
+    dev = torch.device("cuda")
+    Ad = amg._DevMat(A.indptr, A.indices, 500, dev, prec)
+    Ad.data.copy_(torch.as_tensor(A.data))
+    Pd = amg._DevMat(P.indptr, P.indices, 80, dev, prec)
+    Pd.data.copy_(torch.as_tensor(P.data))
+    x, r, w = (rng.standard_normal(500) for _ in range(3))
+    xc = rng.standard_normal(80)
+    T = lambda v: torch.as_tensor(v).to(dev)  # noqa: E731
+    y = torch.empty(500, dtype=torch.float64, device=dev)
+    tol = dict(rtol=1e-12, atol=1e-12)
+    Ad.op(amg._SWEEP_FIRST, None, T(r), T(w), None, y)
+    x1 = w * r
+    np.testing.assert_allclose(y.cpu().numpy(), x1 + w * (r - A @ x1), **tol)
+    Ad.op(amg._SWEEP, T(x), T(r), T(w), None, y)
+    np.testing.assert_allclose(y.cpu().numpy(), x + w * (r - A @ x), **tol)
+    Ad.op(amg._RESIDUAL, T(x), T(r), None, None, y)
+    np.testing.assert_allclose(y.cpu().numpy(), r - A @ x, **tol)
+    yy = T(x)
+    Pd.op(amg._ADD_PRODUCT, T(xc), None, None, yy, yy)
+    np.testing.assert_allclose(yy.cpu().numpy(), x + P @ xc, **tol)
+    from fenics_nonlinear_energies_amd.native import FneError
+
+    with pytest.raises(FneError):
+        Ad.op(amg._SWEEP, yy, T(r), T(w), None, yy)  # in-place sweep is rejected
