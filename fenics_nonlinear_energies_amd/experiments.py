@@ -52,7 +52,7 @@ def he_predictor(p, asm, net, mode, total_angle):
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
                        ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=60, lr=2e-3,
-                       seed=0, guard=True, newton_kwargs=None, log=print):
+                       seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -84,9 +84,16 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1],
            "train_time_s": t_train, "data_gen_time_s": t_gen, "variants": {}}
     E, nu = eval_material
-    for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
-                             ("learned_fp32", "learned", "fp32")):
-        p, asm = _he_assembler(*eval_mesh, E, nu, prec, ksp_max_it, ksp_type, pc_type)
+    # (name, initial guess, Hessian precision, keep the PC, rebuild above this many its, KSP cap)
+    variants = [("previous_fp64", "previous", "fp64", pc_reuse, pc_refresh_its, ksp_max_it),
+                ("extrapolate_fp32", "extrapolate", "fp32", pc_reuse, pc_refresh_its, ksp_max_it),
+                ("learned_fp32", "learned", "fp32", pc_reuse, pc_refresh_its, ksp_max_it)]
+    if with_reference:
+        # the reference's "reference" profile on the GPU (hyperelasticity/jax_petsc/solver.py:35-46):
+        # previous state, FP64, KSP capped at 30 iterations, PC kept and rebuilt at the cap
+        variants.insert(0, ("reference_fp64", "previous", "fp64", True, None, 30))
+    for name, mode, prec, reuse, refresh, max_it in variants:
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, max_it, ksp_type, pc_type)
         pred = he_predictor(p, asm, net, mode, total_angle)
         pred.feat_mean, pred.feat_std, pred.targ_std = stats
         if guard:
@@ -94,8 +101,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         recs, x = solve_he_load_steps(
-            asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it, ksp_rtol=ksp_rtol,
-            pc_setup_on_ksp_cap=pc_reuse, pc_refresh_its=pc_refresh_its, newton_kwargs=newton_kwargs,
+            asm, total_steps, total_angle, predictor=pred, ksp_max_it=max_it, ksp_rtol=ksp_rtol,
+            pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh, newton_kwargs=newton_kwargs,
             on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -114,9 +121,12 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
             f"pc_setups={out['variants'][name]['pc_setups']} ({out['variants'][name]['pc_setup_s']:.1f}s)")
         asm.cleanup()
     ref = out["variants"]["previous_fp64"]
+    base = out["variants"].get("reference_fp64")
     for v in out["variants"].values():
         v["energy_rel_diff_vs_fp64"] = abs(v["final_energy"] - ref["final_energy"]) / abs(ref["final_energy"])
         v["speedup_vs_previous_fp64"] = ref["wall_s"] / v["wall_s"]
+        if base is not None:
+            v["speedup_vs_reference_fp64"] = base["wall_s"] / v["wall_s"]
     return out
 
 
@@ -143,7 +153,7 @@ def slope_predictor(p, asm, net, mode, lam_scale=1.0):
 
 def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.0, eval_h=0.5, degree=4,
                           ksp_max_it=2000, ksp_rtol=1e-3, pc_refresh_its=20, hidden=128, layers=4, epochs=60,
-                          lr=2e-3, seed=0, guard=True, newton_kwargs=None, log=print):
+                          lr=2e-3, seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
     """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (FP32)."""
     from .continuation import elastic_initial_guess, run_lambda_continuation
 
@@ -169,9 +179,16 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
     losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
     log(f"trained on {X.shape[0]} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "variants": {}}
-    for name, mode, prec in (("previous_fp64", "previous", "fp64"), ("extrapolate_fp32", "extrapolate", "fp32"),
-                             ("learned_fp32", "learned", "fp32")):
-        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, ksp_max_it)
+    # (name, initial guess, Hessian precision, keep the PC, rebuild above this many its, KSP cap)
+    variants = [("previous_fp64", "previous", "fp64", True, pc_refresh_its, ksp_max_it),
+                ("extrapolate_fp32", "extrapolate", "fp32", True, pc_refresh_its, ksp_max_it),
+                ("learned_fp32", "learned", "fp32", True, pc_refresh_its, ksp_max_it)]
+    if with_reference:
+        # the reference's "reference" profile on the GPU (slope_stability/jax_petsc/solver.py:44-62):
+        # previous state, FP64, CG capped at 200 iterations, PC set up at every Newton iteration
+        variants.insert(0, ("reference_fp64", "previous", "fp64", False, None, 200))
+    for name, mode, prec, reuse, refresh, max_it in variants:
+        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, max_it)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, el_its = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
@@ -179,8 +196,8 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         pred.feat_mean, pred.feat_std, pred.targ_std = stats
         if guard:
             pred.energy_fn = lambda t, a=asm: float(a.energy_device(t).item())
-        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                                       pc_refresh_its=pc_refresh_its, newton_kwargs=newton_kwargs)
+        recs = run_lambda_continuation(asm, lambdas, x, predictor=pred, ksp_rtol=ksp_rtol, ksp_max_it=max_it,
+                                       pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh, newton_kwargs=newton_kwargs)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         out["variants"][name] = {
@@ -197,7 +214,10 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s")
         asm.cleanup()
     ref = out["variants"]["previous_fp64"]
+    base = out["variants"].get("reference_fp64")
     for v in out["variants"].values():
         v["energy_rel_diff_vs_fp64"] = max(abs(a - b) / abs(b) for a, b in zip(v["energies"], ref["energies"]))
         v["speedup_vs_previous_fp64"] = ref["wall_s"] / v["wall_s"]
+        if base is not None:
+            v["speedup_vs_reference_fp64"] = base["wall_s"] / v["wall_s"]
     return out

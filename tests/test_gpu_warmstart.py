@@ -1,7 +1,7 @@
 """Learned warm start on the rotating Neo-Hookean beam: the trained network saves Newton
 iterations over the extrapolated guess (its fallback) at the same converged energy.
 
-Train: 3 load-step runs of a 40x4x4 beam (E, nu from a seeded draw), 24 steps to 2 pi,
+Train: 3 load-step runs of an 80x4x4 beam (E, nu from a seeded draw), 24 steps to 4 pi,
 per-node MLP 10-128-128-128-128-3 trained by the BF16-MFMA kernels.  Evaluate: an unseen
 material on the same beam; the learned guess (energy-guarded) must need fewer Newton
 iterations in total than the extrapolated guess, and the runs end at the same energy
@@ -22,8 +22,8 @@ def test_learned_warm_start_saves_newton_iterations(require_gpu):
     mats = [(float(rng.uniform(1e8, 4e8)), float(rng.uniform(0.25, 0.4))) for _ in range(3)]
     nk = dict(tolf=0.0, tolg=2e-4, tolg_rel=0.0, tolx_rel=0.0, tolx_abs=0.0, maxit=100,
               require_all_convergence=False)
-    out = he_warmstart_study(train_materials=mats, eval_material=(2e8, 0.3), train_mesh=(40, 4, 4),
-                             eval_mesh=(40, 4, 4), total_steps=24, total_angle=2 * np.pi, epochs=40,
+    out = he_warmstart_study(train_materials=mats, eval_material=(2e8, 0.3), train_mesh=(80, 4, 4),
+                             eval_mesh=(80, 4, 4), total_steps=24, total_angle=4 * np.pi, epochs=40,
                              newton_kwargs=nk, log=lambda *a: None)
     v = out["variants"]
     print({k: (x["newton_its"], round(x["wall_s"], 2)) for k, x in v.items()})
