@@ -32,9 +32,18 @@ NEWTON_DEFAULTS = {
 class LinearSolveContext:
     """State shared by the assemble-and-solve callbacks (PC set-up policy, records)."""
 
+    # Eisenstat-Walker choice 2 (Eisenstat & Walker, SIAM J. Sci. Comput. 17 (1996) 16-32):
+    # eta_k = gamma (|g_k| / |g_{k-1}|)^alpha, safeguarded by gamma eta_{k-1}^alpha when that
+    # exceeds 0.1, capped by ksp_rtol, floored by EW_MIN
+    EW_GAMMA, EW_ALPHA, EW_MIN = 0.9, 2.0, 1e-8
+
     def __init__(self, assembler, *, ksp_rtol, ksp_max_it, pc_setup_on_ksp_cap=False, save_linear_timing=False,
-                 keep_pc=False, pc_refresh_its=None):
+                 keep_pc=False, pc_refresh_its=None, ksp_forcing=None):
         self.asm = assembler
+        if ksp_forcing not in (None, "ew"):
+            raise ValueError("ksp_forcing must be None (fixed ksp_rtol) or 'ew' (Eisenstat-Walker)")
+        self.ksp_forcing = ksp_forcing
+        self._ew_prev = None  # (|g|, eta) of the previous solve
         self.ksp_rtol = float(ksp_rtol)
         self.ksp_max_it = int(ksp_max_it)
         self.pc_setup_on_ksp_cap = bool(pc_setup_on_ksp_cap)
@@ -60,6 +69,23 @@ class LinearSolveContext:
         self.force_pc_setup_next = not (self.keep_pc and self._has_pc())
         self.records = []
         self.iters = []
+        self._ew_prev = None
+
+    def _rtol_for(self, rhs) -> float:
+        """KSP relative tolerance of this solve: ksp_rtol, or the Eisenstat-Walker forcing term."""
+        if self.ksp_forcing is None:
+            return self.ksp_rtol
+        gnorm = float(rhs.norm())
+        eta = self.ksp_rtol
+        if self._ew_prev is not None and self._ew_prev[0] > 0.0:
+            g_prev, eta_prev = self._ew_prev
+            eta = self.EW_GAMMA * (gnorm / g_prev) ** self.EW_ALPHA
+            guard = self.EW_GAMMA * eta_prev ** self.EW_ALPHA
+            if guard > 0.1:
+                eta = max(eta, guard)
+            eta = min(max(eta, self.EW_MIN), self.ksp_rtol)
+        self._ew_prev = (gnorm, eta)
+        return eta
 
     def assemble_and_solve(self, vec, rhs, sol, trust_radius=None, assemble=True) -> int:
         asm = self.asm
@@ -71,7 +97,7 @@ class LinearSolveContext:
         if trust_radius is not None:
             ksp.setCGRadius(float(trust_radius))
         ksp.setOperators(asm.A)
-        ksp.setTolerances(rtol=self.ksp_rtol, max_it=self.ksp_max_it)
+        ksp.setTolerances(rtol=self._rtol_for(rhs), max_it=self.ksp_max_it)
         t1 = time.perf_counter()
         if self.pc_setup_on_ksp_cap:
             if self.force_pc_setup_next:
@@ -125,6 +151,7 @@ def solve_newton(
     verbose=False,
     keep_pc=False,
     pc_refresh_its=None,
+    ksp_forcing=None,
     **newton_kwargs,
 ):
     """Run Newton (with the reference's retry schedule) from device vector ``x`` in place.
@@ -132,6 +159,9 @@ def solve_newton(
     ``first_step_laplacian``: compute the first Newton direction with the assembler's
     ``assemble_first_step_hessian`` (the p = 2 Hessian for p-Laplace), the stated policy for
     zero initial guesses where the exact p > 2 Hessian vanishes identically.
+    ``ksp_forcing="ew"``: the KSP tolerance of each Newton iteration follows the
+    Eisenstat-Walker forcing term, capped by ``ksp_rtol`` (the reference uses ``ksp_rtol``
+    throughout, which is the default here too).
     """
     opts = dict(NEWTON_DEFAULTS)
     opts.update(newton_kwargs)
@@ -139,6 +169,7 @@ def solve_newton(
     ctx = LinearSolveContext(
         assembler, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
         save_linear_timing=save_linear_timing, keep_pc=keep_pc, pc_refresh_its=pc_refresh_its,
+        ksp_forcing=ksp_forcing,
     )
     if first_step_laplacian and first_direction_fn is None:
         def first_direction_fn(vec, rhs, sol):

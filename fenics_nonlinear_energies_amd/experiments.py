@@ -52,7 +52,7 @@ def he_predictor(p, asm, net, mode, total_angle):
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
                        ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=60, lr=2e-3,
-                       seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
+                       seed=0, guard=True, with_reference=True, ksp_forcing=None, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -102,7 +102,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         t0 = time.perf_counter()
         recs, x = solve_he_load_steps(
             asm, total_steps, total_angle, predictor=pred, ksp_max_it=max_it, ksp_rtol=ksp_rtol,
-            pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh, newton_kwargs=newton_kwargs,
+            pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh,
+            newton_kwargs=dict(newton_kwargs or {}, **({} if name == "reference_fp64" else {"ksp_forcing": ksp_forcing})),
             on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
