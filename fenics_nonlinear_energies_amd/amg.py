@@ -20,7 +20,7 @@ Set-up in two phases.  Symbolic (host, scipy, once per Hessian pattern, done by 
 assembler at construction): the patterns of P, A P, P^T and P^T A P on every level, the
 transpose permutations and the diagonal positions.  Numeric (device, every set-up): the
 per-aggregate QR of the near-nullspace (``fne_aggregate_qr``; the near-nullspace may follow
-the current state), the spectral radius by power iteration, and the three products per level
+the current state), the spectral radius by a short Lanczos run, and the three products per level
 on the fixed-pattern product kernel (``fne_spgemm_numeric``); only the coarsest operator goes
 to the host for its dense inverse.
 Level operators and transfers are kept in FP64 for the products and copied to the
@@ -188,8 +188,8 @@ def tentative_prolongator(B: np.ndarray, agg_dofs: int):
 class SmoothedAggregationPC:
     """``pc_type="gamg"``: smoothed-aggregation AMG V-cycle on the device."""
 
-    POWER_ITS = 12  # power iterations for rho(D^-1 A) at the first set-up
-    POWER_ITS_WARM = 4  # at later set-ups, started from the previous vector
+    LANCZOS_STEPS = 12  # Lanczos steps for rho(D^-1 A) at every set-up
+    RHO_SAFETY = 1.1  # the Ritz value from below is raised by this factor
 
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
                  max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
@@ -346,22 +346,10 @@ class SmoothedAggregationPC:
             d = A0.diagonal() if lvl == 0 else a_dat[L["diag"]]
             d = torch.where(d == 0.0, torch.ones_like(d), d)
             dinv = 1.0 / d
-            # rho(D^-1 A) by power iteration on the device (no host synchronisation), warm-started
-            # from the previous set-up's vector
-            v = L["v0"] / torch.linalg.vector_norm(L["v0"])
-            y = torch.empty_like(v)
-            lam = None
-            for _ in range(self.POWER_ITS_WARM if L.get("warm") else self.POWER_ITS):
-                if lvl == 0:
-                    A0.mult_tensor(v, y)
-                else:
-                    self.levels[lvl - 1]["Ac64"].mult(v, y)
-                y.mul_(dinv)
-                lam = torch.linalg.vector_norm(y)
-                v = y / lam
-            L["v0"].copy_(v)
-            L["warm"] = True
-            omega = 4.0 / (3.0 * 1.05 * lam)
+            # rho(D^-1 A): largest Ritz value of a Lanczos run on D^-1/2 A D^-1/2, times a safety
+            # factor, so that the smoother's omega rho stays well below 2
+            lam = self._lanczos_lambda_max(lvl, A0, dinv, L["v0"])
+            omega = 4.0 / (3.0 * self.RHO_SAFETY * lam)
             L["wd"].copy_(omega * dinv)
             # P = P_tent - omega D^-1 (A P_tent)
             Pt, P64 = L["Pt"], L["P64"]
@@ -400,6 +388,39 @@ class SmoothedAggregationPC:
         self.setup_time += time.perf_counter() - t_setup
         if self.use_graph and self._graph is None and torch.device(device).type == "cuda":
             self._capture(n, device)
+
+    def _lanczos_lambda_max(self, lvl: int, A0, dinv: torch.Tensor, v0: torch.Tensor) -> float:
+        """Largest eigenvalue of D^-1 A (similar to S = D^-1/2 A D^-1/2) from LANCZOS_STEPS
+        Lanczos steps on S from the fixed start vector v0; the recurrence stays on the device and
+        one transfer brings the tridiagonal coefficients to the host.  A Krylov estimate: the
+        power iteration it replaces could stay 20-30 % low after a few steps once plasticity
+        changed the spectrum, which made the smoother (and the V-cycle) indefinite."""
+        s = torch.sqrt(dinv)
+        mult = A0.mult_tensor if lvl == 0 else self.levels[lvl - 1]["Ac64"].mult
+        v = v0 / torch.linalg.vector_norm(v0)
+        v_prev = torch.zeros_like(v)
+        beta = torch.zeros((), dtype=v.dtype, device=v.device)
+        t = torch.empty_like(v)
+        y = torch.empty_like(v)
+        coef = []
+        for _ in range(self.LANCZOS_STEPS):
+            torch.mul(s, v, out=t)
+            mult(t, y)
+            w = s * y - beta * v_prev
+            alpha = torch.dot(w, v)
+            w = w - alpha * v
+            beta = torch.linalg.vector_norm(w)
+            coef.append(alpha)
+            coef.append(beta)
+            v_prev, v = v, w / torch.clamp(beta, min=1e-300)
+        c = torch.stack(coef).cpu().numpy()
+        a, b = c[0::2], c[1::2]
+        k = a.size
+        if not np.all(np.isfinite(c)):
+            raise FloatingPointError("non-finite Lanczos coefficients in the multigrid set-up")
+        lam = float(sla.eigh_tridiagonal(a, b[: k - 1], eigvals_only=True, select="i",
+                                         select_range=(k - 1, k - 1))[0])
+        return max(lam, 1e-300)
 
     # ---------------------------------------------------------------------------------------
     def _capture(self, n: int, device) -> None:

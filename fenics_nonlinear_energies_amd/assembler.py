@@ -83,6 +83,7 @@ class ReorderedElementAssemblerBase:
         petsc_log_events=False,
         memory_guard_total_gib=None,
         hessian_precision="fp64",
+        pc_precision=None,
         assembly_mode="staged",
         rows_per_patch=None,
         chunk=None,
@@ -174,6 +175,9 @@ class ReorderedElementAssemblerBase:
         self.A = DeviceCSR(self.layout.indptr, self.layout.indices, self.layout.n_free, self.hessian_precision, self.device)
         self.ksp = KSP(ksp_type=ksp_type, pc_type=pc_type, rtol=ksp_rtol, max_it=ksp_max_it)
         self.ksp.near_nullspace = self._near_nullspace_reordered()
+        # precision of the multigrid hierarchy (None: the Hessian's); "fp32" with an FP64
+        # Hessian keeps the Krylov operator exact and stores only the preconditioner in FP32
+        self.ksp.pc_precision = pc_precision
         self._setup_timings["matrix_create"] = time.perf_counter() - t0
         # multigrid symbolic phase (patterns of the Galerkin products): mesh preprocessing,
         # done once here like the layout and the assembly plans

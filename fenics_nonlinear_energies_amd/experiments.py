@@ -135,13 +135,14 @@ SLOPE_GEOMETRY = dict(x1=20.0, x2=20.0, x3=20.0, y1=10.0, y2=10.0)  # 60 x 20 m,
 SLOPE_MATERIAL = dict(E=20000.0, nu=0.49, c0=6.0, phi_deg=45.0, psi_deg=0.0, gamma=20.0)
 
 
-def _slope_assembler(h, degree, material, precision, ksp_max_it, pc_type="gamg"):
+def _slope_assembler(h, degree, material, precision, ksp_max_it, pc_type="gamg", pc_precision=None):
     from .problems.slope_stability import SlopeStabilityReorderedElementAssembler, build_slope_problem
 
     p, _, _ = build_slope_problem(h, degree=degree, geometry=SLOPE_GEOMETRY, material=material, with_adjacency=False)
     p["phi_deg_raw"] = float(p["phi_deg"])
     asm = SlopeStabilityReorderedElementAssembler(p, None, None, hessian_precision=precision, ksp_type="cg",
-                                                  pc_type=pc_type, ksp_rtol=1e-3, ksp_max_it=ksp_max_it)
+                                                  pc_type=pc_type, ksp_rtol=1e-3, ksp_max_it=ksp_max_it,
+                                                  pc_precision=pc_precision)
     return p, asm
 
 
@@ -153,17 +154,28 @@ def slope_predictor(p, asm, net, mode, lam_scale=1.0):
 
 
 def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.0, eval_h=0.5, degree=4,
-                          ksp_max_it=2000, ksp_rtol=1e-3, pc_refresh_its=20, hidden=128, layers=4, epochs=60,
-                          lr=2e-3, seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
-    """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (FP32)."""
+                          ksp_max_it=200, ksp_rtol=1e-3, pc_refresh_its=None, mixed="fp32pc", hidden=128, layers=4,
+                          epochs=60, lr=2e-3, seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
+    """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (mixed).
+
+    ``mixed="fp32pc"``: FP64 Hessian and Krylov operator, multigrid stored in FP32 (the P4
+    nu = 0.49 Hessian is too ill-conditioned for FP32 storage at ~1.5M DOFs: rounding its
+    entries makes it indefinite and CG breaks down); ``mixed="fp32"``: FP32 Hessian and
+    multigrid.  All but ``reference_fp64`` keep the preconditioner and rebuild it when CG hits
+    ``ksp_max_it`` (or needs more than ``pc_refresh_its``).
+    """
     from .continuation import elastic_initial_guess, run_lambda_continuation
+
+    if mixed not in ("fp32", "fp32pc"):
+        raise ValueError("mixed must be 'fp32' or 'fp32pc'")
+    mix_h, mix_pc = ("fp32", None) if mixed == "fp32" else ("fp64", "fp32")
 
     dims = [7] + [hidden] * layers + [2]
     net = NodeMLP(dims, seed=seed)
     feats, targs = [], []
     pred = None
     for mat in train_materials:
-        p, asm = _slope_assembler(train_h, degree, mat, "fp32", ksp_max_it)
+        p, asm = _slope_assembler(train_h, degree, mat, mix_h, ksp_max_it, pc_precision=mix_pc)
         x, _ = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
         pred = slope_predictor(p, asm, None, "extrapolate")
         holder = {}
@@ -180,16 +192,16 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
     losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
     log(f"trained on {X.shape[0]} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "variants": {}}
-    # (name, initial guess, Hessian precision, keep the PC, rebuild above this many its, KSP cap)
-    variants = [("previous_fp64", "previous", "fp64", True, pc_refresh_its, ksp_max_it),
-                ("extrapolate_fp32", "extrapolate", "fp32", True, pc_refresh_its, ksp_max_it),
-                ("learned_fp32", "learned", "fp32", True, pc_refresh_its, ksp_max_it)]
+    # (name, initial guess, Hessian precision, PC precision, keep the PC, rebuild above, KSP cap)
+    variants = [("previous_fp64", "previous", "fp64", None, True, pc_refresh_its, ksp_max_it),
+                (f"extrapolate_{mixed}", "extrapolate", mix_h, mix_pc, True, pc_refresh_its, ksp_max_it),
+                (f"learned_{mixed}", "learned", mix_h, mix_pc, True, pc_refresh_its, ksp_max_it)]
     if with_reference:
         # the reference's "reference" profile on the GPU (slope_stability/jax_petsc/solver.py:44-62):
         # previous state, FP64, CG capped at 200 iterations, PC set up at every Newton iteration
-        variants.insert(0, ("reference_fp64", "previous", "fp64", False, None, 200))
-    for name, mode, prec, reuse, refresh, max_it in variants:
-        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, max_it)
+        variants.insert(0, ("reference_fp64", "previous", "fp64", None, False, None, 200))
+    for name, mode, prec, pc_prec, reuse, refresh, max_it in variants:
+        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, max_it, pc_precision=pc_prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, el_its = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)

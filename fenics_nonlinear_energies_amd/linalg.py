@@ -140,6 +140,7 @@ class KSP:
         self.check_every_graph_pc = 1  # the same with the multigrid graph (a wasted cycle costs more)
         self.near_nullspace = None  # (n, k) host array for pc_type "gamg" (rigid modes, constants)
         self.near_nullspace_fn = None  # optional callable -> (n, k) array at the current state, or None
+        self.pc_precision = None  # multigrid storage precision ("fp64" / "fp32"); None: the operator's
         self._amg = None
 
     # PETSc-style configuration -----------------------------------------------------------
@@ -176,12 +177,13 @@ class KSP:
         elif self.pc_type in ("gamg", "hypre"):
             from .amg import SmoothedAggregationPC
 
-            if self._amg is None or self._amg.precision != self.A.precision:
+            prec = self.pc_precision or self.A.precision
+            if self._amg is None or self._amg.precision != prec:
                 prepared = getattr(self, "_amg_prepared", None)
-                if prepared is not None and prepared.precision == self.A.precision:
+                if prepared is not None and prepared.precision == prec:
                     self._amg = prepared
                 else:
-                    self._amg = SmoothedAggregationPC(self.near_nullspace, precision=self.A.precision)
+                    self._amg = SmoothedAggregationPC(self.near_nullspace, precision=prec)
             B = self.near_nullspace_fn() if self.near_nullspace_fn is not None else None
             self._amg.setup(self.A, near_nullspace=B)
             self._inv_diag = None
@@ -197,7 +199,7 @@ class KSP:
             return 0.0
         from .amg import SmoothedAggregationPC
 
-        pc = SmoothedAggregationPC(self.near_nullspace, precision=A.precision)
+        pc = SmoothedAggregationPC(self.near_nullspace, precision=self.pc_precision or A.precision)
         k = 1 if self.near_nullspace is None else int(np.asarray(self.near_nullspace).shape[1])
         pc.prepare(A, k)
         self._amg_prepared = pc
