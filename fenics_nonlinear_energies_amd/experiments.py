@@ -51,7 +51,7 @@ def he_predictor(p, asm, net, mode, total_angle):
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
-                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=60, lr=2e-3,
+                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=200, lr=2e-3,
                        seed=0, guard=True, with_reference=True, ksp_forcing=None, newton_kwargs=None, log=print):
     from .problems.hyperelasticity import solve_he_load_steps
 
