@@ -8,10 +8,13 @@ assembler's CSR by the HIP kernels and solved by the GPU KSP.
 
 from __future__ import annotations
 
+import math
 import time
 
 import numpy as np
+import torch
 
+from . import native
 from .minimizers import newton
 from .repair import build_retry_attempts, needs_solver_repair
 
@@ -27,6 +30,51 @@ NEWTON_DEFAULTS = {
     "require_all_convergence": True,
     "fail_on_nonfinite": True,
 }
+
+
+class DeviceGoldenSearch:
+    """Golden-section line search on the device (``fne_golden_trial`` / ``fne_energy`` /
+    ``fne_golden_step`` per point): the same points and comparisons as
+    ``minimizers.golden_section_search`` (``core/petsc/minimizers.py:14-64``), with one host
+    synchronisation per search instead of one per energy."""
+
+    def __init__(self, assembler):
+        self.asm = assembler
+        dev = assembler.device
+        n_state = int(native.lib().fne_golden_state_len())
+        self.state = torch.zeros(n_state, dtype=torch.float64, device=dev)
+        self.trial = torch.empty(assembler.layout.n_free, dtype=torch.float64, device=dev)
+        self.energy = torch.zeros(1, dtype=torch.float64, device=dev)
+        self._init = torch.zeros(n_state, dtype=torch.float64)
+        self.searches = 0
+
+    def _point(self, xt: torch.Tensor, pt: torch.Tensor) -> None:
+        lib = native.lib()
+        sp = native.stream_ptr()
+        native.check(lib.fne_golden_trial(int(xt.numel()), native.ptr(xt), native.ptr(pt), native.ptr(self.state),
+                                          native.ptr(self.trial), sp), "fne_golden_trial")
+        self.asm.energy_device(self.trial, out=self.energy)
+        native.check(lib.fne_golden_step(native.ptr(self.state), native.ptr(self.energy), sp), "fne_golden_step")
+
+    def __call__(self, x_prev, p, lo: float, hi: float, tol: float):
+        lo, hi, tol = float(lo), float(hi), float(tol)
+        gamma = 0.5 + math.sqrt(5.0) / 2.0
+        init = self._init
+        init.zero_()
+        init[0], init[1], init[10], init[13] = lo, hi, lo, tol  # lo, hi, first point, tol
+        self.state.copy_(init)
+        # 4 initial points, the shrink steps until hi - lo <= tol (+2 for rounding), the midpoint;
+        # steps after the search has finished are no-ops
+        n_loop = int(math.ceil(math.log(tol / (hi - lo)) / math.log(1.0 / gamma))) + 2 if hi - lo > tol else 0
+        xt, pt = x_prev.t, p.t
+        for _ in range(4 + max(n_loop, 0) + 1):
+            self._point(xt, pt)
+        st = self.state.cpu().numpy()
+        while st[11] < 7.0:  # not finished (rounding made the bracket shrink more slowly)
+            self._point(xt, pt)
+            st = self.state.cpu().numpy()
+        self.searches += 1
+        return float(st[8]), float(st[9]), int(st[12])
 
 
 class LinearSolveContext:
@@ -152,6 +200,7 @@ def solve_newton(
     keep_pc=False,
     pc_refresh_its=None,
     ksp_forcing=None,
+    device_linesearch=True,
     **newton_kwargs,
 ):
     """Run Newton (with the reference's retry schedule) from device vector ``x`` in place.
@@ -159,6 +208,8 @@ def solve_newton(
     ``first_step_laplacian``: compute the first Newton direction with the assembler's
     ``assemble_first_step_hessian`` (the p = 2 Hessian for p-Laplace), the stated policy for
     zero initial guesses where the exact p > 2 Hessian vanishes identically.
+    ``device_linesearch``: run the golden-section line search on the device
+    (``DeviceGoldenSearch``; same points and comparisons, one synchronisation per search).
     ``ksp_forcing="ew"``: the KSP tolerance of each Newton iteration follows the
     Eisenstat-Walker forcing term, capped by ``ksp_rtol`` (the reference uses ``ksp_rtol``
     throughout, which is the default here too).
@@ -180,6 +231,11 @@ def solve_newton(
     attempts = build_retry_attempts(
         retry_on_failure=bool(retry_on_failure), linesearch_interval=ls, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it
     )
+    dls = None
+    if device_linesearch and getattr(assembler, "device", None) is not None and hasattr(assembler, "energy_device"):
+        dls = getattr(assembler, "_device_golden", None)
+        if dls is None:
+            dls = assembler._device_golden = DeviceGoldenSearch(assembler)
     x_initial = x.copy()
     result = None
     used = attempts[0]
@@ -201,6 +257,7 @@ def solve_newton(
             save_history=save_history,
             trust_region=use_trust_region,
             first_direction_fn=first_direction_fn,
+            device_linesearch=dls,
             **opts,
         )
         t_solve = time.perf_counter() - t0

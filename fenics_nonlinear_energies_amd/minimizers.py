@@ -183,12 +183,17 @@ def newton(
     step_time_limit_s=None,
     iteration_callback=None,
     first_direction_fn=None,
+    device_linesearch=None,
 ):
     """Newton's method for energy minimisation; returns the reference's result dict.
 
     ``first_direction_fn(x, rhs, sol) -> ksp_its`` (optional, not in the reference) replaces
     ``hessian_solve_fn`` on the first iteration only; it carries the stated first-step policy
     for starts where the exact Hessian vanishes (p-Laplace, p > 2, from u = 0).
+    ``device_linesearch(x_prev, p, lo, hi, tol) -> (alpha, J(x_prev + alpha p), n_evals)``
+    (optional) runs the golden-section search on the device with the same points and
+    comparisons as ``golden_section_search``; used only without ``project_fn`` /
+    ``ghost_update_fn``.
     """
     trust_shrink, trust_expand, trust_eta_shrink, trust_eta_expand, trust_max_reject = _sanitize_trust(
         trust_shrink, trust_expand, trust_eta_shrink, trust_eta_expand, trust_max_reject
@@ -354,6 +359,11 @@ def newton(
             )
             it["ls_evals"] += n_rep
             it["ls_repaired"] = rep
+            if device_linesearch is not None and project_fn is None and ghost_update_fn is None:
+                alpha, val, n_ev = device_linesearch(x_prev, p, alpha_lo, alpha_hi, linesearch_tol)
+                it["ls_evals"] += n_ev
+                x_trial.waxpy(alpha, p, x_prev)  # the accepted point, as energy_at(alpha) leaves it
+                return alpha, val if np.isfinite(val) else np.inf
             alpha, n_ev = golden_section_search(energy_at, alpha_lo, alpha_hi, linesearch_tol)
             it["ls_evals"] += n_ev
             return alpha, energy_at(alpha)

@@ -102,6 +102,9 @@ _SIGNATURES = [
         C.c_int,
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
     ),
+    ("fne_golden_state_len", C.c_int, []),
+    ("fne_golden_trial", C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fne_golden_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     (
         "fne_aggregate_qr",
         C.c_int,

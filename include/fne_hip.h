@@ -180,6 +180,22 @@ int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, 
  * up per Newton iteration through ksp.setOperators/solve in core/petsc/scalar_problem_driver.py).
  */
 /*
+ * Golden-section line search on the device (replaces golden_section_search,
+ * core/petsc/minimizers.py:14-64, with the same points and comparisons).  state: device FP64
+ * array of fne_golden_state_len() slots, initialised by the caller to zero except
+ * state[0] = lo, state[1] = hi, state[10] = lo (first point), state[13] = tol.  Per point:
+ * fne_golden_trial writes trial = x + alpha p for the state's next alpha (no-op once finished),
+ * the caller evaluates the energy of trial into f_new (e.g. fne_energy), and fne_golden_step
+ * consumes it and advances the state.  Finished when state[11] >= 7; then state[8] is the chosen
+ * alpha, state[9] its energy (non-finite energies count as +inf) and state[12] the number of
+ * energies used.  Extra steps after the finish are no-ops.
+ */
+int fne_golden_state_len(void);
+int fne_golden_trial(int32_t n, const double* x, const double* p, const double* state, double* trial,
+                     fne_stream_t stream);
+int fne_golden_step(double* state, const double* f_new, fne_stream_t stream);
+
+/*
  * Tentative prolongator of the multigrid set-up: thin QR of every aggregate's rows of the
  * near-nullspace B (n x k row-major FP64, k in {1, 2, 3, 6}); aggregate a holds rows
  * [a*agg, min(n, (a+1)*agg)), k <= agg <= 256.  q_vals receives the Q blocks in CSR order
