@@ -584,7 +584,186 @@ int launch_mc(fne_plan* plan, FneOp op, const FneCall& c) {
   return FNE_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Matrix-free Hessian products (slope_stability/jax_petsc/reordered_element_assembler.py:313-562).
+// AUX = 1: element vectors H_e v_e = sum_q w_q B_q^T D_q (B_q v_e) with D_q the tangent of the
+// return mapping at the state x (the matrix the element kernel assembles); AUX = 2: the element
+// diagonals diag(H_e)_a = sum_q w_q b_a^T D_q b_a.  Both are written to the element-vector
+// scratch and summed per row by the row gather, like the gradient.  FP64 throughout.
+// ---------------------------------------------------------------------------------------------
+struct McAuxLds {
+  double u[kMcEpw][32];
+  double v[kMcEpw][32];
+  double geo[kMcEpw][8];
+  double P[kMcEpw][kMcMaxQ][4];  // AUX 1: w * (D B v)_{i a} K[a][g]
+  double D[kMcEpw][kMcMaxQ][6];  // AUX 2: w * D
+};
+
+template <int NN, int AUX>
+__global__ __launch_bounds__(kBlock) void mc_aux_kernel(fne_plan_desc d, int mode, const double* __restrict__ x,
+                                                        const double* __restrict__ dvals,
+                                                        const double* __restrict__ vin, double* __restrict__ gbuf) {
+  constexpr int NL = 2 * NN;
+  __shared__ double sTab[kMcMaxQ * 16 * 2];
+  __shared__ double sW[kMcMaxQ];
+  __shared__ McAuxLds sWave[kMcWaves];
+  const int nq = d.nq;
+  for (int i = threadIdx.x; i < nq * NN * 2; i += blockDim.x) sTab[i] = d.qdata[nq + i];
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) sW[i] = d.qdata[i];
+  __syncthreads();
+  const McConst k = mc_const(d.param, mode);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  McAuxLds& L = sWave[wave];
+  const int ne = d.n_elem;
+  const bool has_eps = d.n_geo >= 5 + 3 * nq;
+  const int nwaves = gridDim.x * kMcWaves;
+  for (int base = (blockIdx.x * kMcWaves + wave) * kMcEpw; base < ne; base += nwaves * kMcEpw) {
+    for (int i = lane; i < kMcEpw * NL; i += 64) {
+      const int t = i / NL, c = i - t * NL;
+      const int e = base + t;
+      const int code = e < ne ? d.edof[size_t(c) * ne + e] : d.n_free;
+      L.u[t][c] = e < ne ? gather_dof(code, d.n_free, x, dvals) : 0.0;
+      if (AUX == 1) L.v[t][c] = (e < ne && code < d.n_free) ? vin[code] : 0.0;  // v vanishes on constrained DOFs
+    }
+    if (lane < kMcEpw * 5) {
+      const int t = lane / 5, c = lane - 5 * t;
+      const int e = base + t;
+      L.geo[t][c] = e < ne ? d.egeo[size_t(c) * ne + e] : (c == 4 ? 0.0 : 1.0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < kMcEpw * nq) {
+      const int t = lane / nq, q = lane - t * nq;
+      const int e = base + t;
+      if (e < ne) {
+        double U00 = 0.0, U01 = 0.0, U10 = 0.0, U11 = 0.0, V00 = 0.0, V01 = 0.0, V10 = 0.0, V11 = 0.0;
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+          const double g0 = sTab[(q * NN + n) * 2 + 0], g1 = sTab[(q * NN + n) * 2 + 1];
+          U00 += L.u[t][2 * n] * g0;
+          U01 += L.u[t][2 * n] * g1;
+          U10 += L.u[t][2 * n + 1] * g0;
+          U11 += L.u[t][2 * n + 1] * g1;
+          if (AUX == 1) {
+            V00 += L.v[t][2 * n] * g0;
+            V01 += L.v[t][2 * n] * g1;
+            V10 += L.v[t][2 * n + 1] * g0;
+            V11 += L.v[t][2 * n + 1] * g1;
+          }
+        }
+        const double K00 = L.geo[t][0], K01 = L.geo[t][1], K10 = L.geo[t][2], K11 = L.geo[t][3];
+        double e0 = K00 * U00 + K01 * U01;
+        double e1 = K10 * U10 + K11 * U11;
+        double e2 = (K10 * U00 + K11 * U01) + (K00 * U10 + K01 * U11);
+        if (has_eps) {
+          const size_t o = size_t(5 + 3 * q) * ne + e;
+          e0 -= d.egeo[o];
+          e1 -= d.egeo[o + ne];
+          e2 -= d.egeo[o + 2 * size_t(ne)];
+        }
+        double sig[3], D[6];
+        mc_point<true>(k, e0, e1, e2, sig, D);
+        const double w = sW[q] * L.geo[t][4];
+        if (AUX == 1) {
+          const double v0 = K00 * V00 + K01 * V01;
+          const double v1 = K10 * V10 + K11 * V11;
+          const double v2 = (K10 * V00 + K11 * V01) + (K00 * V10 + K01 * V11);
+          const double s0 = w * (D[0] * v0 + D[1] * v1 + D[2] * v2);
+          const double s1 = w * (D[1] * v0 + D[3] * v1 + D[4] * v2);
+          const double s2 = w * (D[2] * v0 + D[4] * v1 + D[5] * v2);
+          L.P[t][q][0] = s0 * K00 + s2 * K10;
+          L.P[t][q][1] = s0 * K01 + s2 * K11;
+          L.P[t][q][2] = s2 * K00 + s1 * K10;
+          L.P[t][q][3] = s2 * K01 + s1 * K11;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) L.D[t][q][i] = w * D[i];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int i = lane; i < kMcEpw * NL; i += 64) {
+      const int t = i / NL, rr = i - t * NL;
+      const int e = base + t;
+      if (e >= ne) continue;
+      const int n = rr >> 1, comp = rr & 1;
+      double s = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const double g0 = sTab[(q * NN + n) * 2 + 0], g1 = sTab[(q * NN + n) * 2 + 1];
+        if (AUX == 1) {
+          s += g0 * L.P[t][q][2 * comp] + g1 * L.P[t][q][2 * comp + 1];
+        } else {
+          const double gx = L.geo[t][0] * g0 + L.geo[t][1] * g1;
+          const double gy = L.geo[t][2] * g0 + L.geo[t][3] * g1;
+          const double* Dq = L.D[t][q];
+          // b = (gx, 0, gy) for an x component, (0, gy, gx) for a y component
+          s += comp == 0 ? Dq[0] * gx * gx + 2.0 * Dq[2] * gx * gy + Dq[5] * gy * gy
+                         : Dq[3] * gy * gy + 2.0 * Dq[4] * gx * gy + Dq[5] * gx * gx;
+        }
+      }
+      gbuf[size_t(e) * 32 + rr] = s;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+template <int NN, int AUX>
+int launch_mc_aux(fne_plan* plan, const double* x, const double* dvals, const double* v, double* out,
+                  hipStream_t stream) {
+  const fne_plan_desc& d = plan->d;
+  int nb = (d.n_elem + kMcWaves * kMcEpw - 1) / (kMcWaves * kMcEpw);
+  if (nb > kEnergyBlocks) nb = kEnergyBlocks;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL((mc_aux_kernel<NN, AUX>), dim3(nb), dim3(kBlock), 0, stream, d, plan->mode, x, dvals, v,
+                     plan->mc_g);
+  FNE_LAUNCH_CHECK();
+  FneCall c{};
+  c.f = nullptr;
+  c.g = out;
+  c.precision = FNE_FP64;
+  c.stream = stream;
+  return fne_launch_row_gather(plan, true, false, c);
+}
+
+int mc_aux(fne_plan* plan, int aux, const double* x, const double* dvals, const double* v, double* out,
+           fne_stream_t stream) {
+  if (!plan || !x || !out || (aux == 1 && (!v || v == out))) {
+    fne_set_error("invalid arguments");
+    return FNE_EINVAL;
+  }
+  const fne_plan_desc& d = plan->d;
+  if (d.kind != FNE_KIND_MOHR_COULOMB) {
+    fne_set_error("matrix-free Hessian products are implemented for the Mohr-Coulomb family");
+    return FNE_EUNSUPPORTED;
+  }
+  if (!plan->mc_g || !d.inc_ptr || !d.inc) {
+    fne_set_error("the plan needs FNE_FLAG_ROW_GATHER (row incidences and element scratch)");
+    return FNE_EINVAL;
+  }
+  if (d.n_total > 0 && !dvals) {
+    fne_set_error("Dirichlet values missing");
+    return FNE_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (d.n_loc == 12) return aux == 1 ? launch_mc_aux<6, 1>(plan, x, dvals, v, out, s)
+                                     : launch_mc_aux<6, 2>(plan, x, dvals, v, out, s);
+  return aux == 1 ? launch_mc_aux<15, 1>(plan, x, dvals, v, out, s) : launch_mc_aux<15, 2>(plan, x, dvals, v, out, s);
+}
+
 }  // namespace
+
+extern "C" int fne_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y,
+                       fne_stream_t stream) {
+  return mc_aux(plan, 1, x, dvals, v, y, stream);
+}
+
+extern "C" int fne_hessian_diag(fne_plan* plan, const double* x, const double* dvals, double* diag,
+                                fne_stream_t stream) {
+  return mc_aux(plan, 2, x, dvals, nullptr, diag, stream);
+}
 
 int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c) {
   const fne_plan_desc& d = plan->d;

@@ -14,6 +14,10 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("FNE_HIP_LIB", PKG_DIR / "_lib" / "libfne_hip.so"))
 
 FNE_OK = 0
+FNE_EINVAL = -1
+FNE_EHIP = -2
+FNE_ENOMEM = -3
+FNE_EUNSUPPORTED = -4
 FNE_KIND_PLAPLACE = 1
 FNE_KIND_GINZBURG_LANDAU = 2
 FNE_KIND_NEOHOOKEAN = 3
@@ -92,6 +96,8 @@ _SIGNATURES = [
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     ),
+    ("fne_hvp", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fne_hessian_diag", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     (
         "fne_spmv",
         C.c_int,

@@ -307,3 +307,18 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
             return self.assemble_hessian(u_owned)
         finally:
             self.set_constitutive_mode(prev)
+
+    # matrix-free Hessian operator (slope_stability/jax_petsc/reordered_element_assembler.py:313-562)
+    def hvp_device(self, x, v, y) -> None:
+        """y = H(x) v without assembling H (``fne_hvp``); device FP64 tensors on the free DOFs."""
+        from .. import native
+
+        native.check(native.lib().fne_hvp(self._plan, native.ptr(x), native.ptr(self.dvals), native.ptr(v),
+                                          native.ptr(y), native.stream_ptr()), "fne_hvp")
+
+    def hessian_diagonal_device(self, x, diag) -> None:
+        """diag = diagonal of H(x) without assembling H (``fne_hessian_diag``)."""
+        from .. import native
+
+        native.check(native.lib().fne_hessian_diag(self._plan, native.ptr(x), native.ptr(self.dvals),
+                                                   native.ptr(diag), native.stream_ptr()), "fne_hessian_diag")

@@ -160,6 +160,18 @@ int fne_element_eval(fne_plan* plan, const double* x, const double* dvals, doubl
                      double* ge_out, double* he_out, fne_stream_t stream);
 
 /*
+ * Matrix-free Hessian products, Mohr-Coulomb P2/P4 plans with FNE_FLAG_ROW_GATHER (FP64):
+ * fne_hvp: y = H(x) v over the free DOFs (v is taken as zero on constrained DOFs);
+ * fne_hessian_diag: diag = diagonal of H(x).  H(x) is the matrix fne_hessian assembles (the
+ * return-mapping tangent at x in the plan's constitutive mode), applied element by element
+ * without forming it.  Replace the matrix-free operator and its diagonal
+ * (slope_stability/jax_petsc/reordered_element_assembler.py:313-562).  Other families:
+ * FNE_EUNSUPPORTED.  y and v must not alias.
+ */
+int fne_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, fne_stream_t stream);
+int fne_hessian_diag(fne_plan* plan, const double* x, const double* dvals, double* diag, fne_stream_t stream);
+
+/*
  * CSR SpMV y = A x, A values FP64 or FP32, x/y FP64.  Replaces PETSc MatMult
  * (hessian_matvec_fn = A.mult, core/petsc/scalar_problem_driver.py:294).
  */
