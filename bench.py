@@ -109,11 +109,13 @@ def secondary_workloads() -> list[dict]:
     g, y = x.duplicate(), x.duplicate()
     ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t))
     sp = _time_ms(lambda: asm.A.mult(x, y))
+    hv = x.t.clone()
+    mf = _time_ms(lambda: asm.hvp_device(x.t, y.t, hv))  # matrix-free H v at the same state (FP64)
     ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
     out.append({"workload": "Mohr-Coulomb P4 slope 60x20 m, h=1/7 (plastic state)", "n_elem": ne, "n_free": nf,
                 "nnz": int(nnz), "assembly_ms": ms, "elem_per_s": ne / (ms * 1e-3), "spmv_ms": sp,
                 "spmv_GBs": (8 * nnz + 20 * nf) / (sp * 1e-3) / 1e9,
-                "mfma_TFLOPs": ne * 112.9e3 / (ms * 1e-3) / 1e12})
+                "mfma_TFLOPs": ne * 112.9e3 / (ms * 1e-3) / 1e12, "matfree_hvp_ms": mf})
     asm.cleanup()
     del asm
     p, adj, u0 = build_beam_problem(160, 8, 8)
