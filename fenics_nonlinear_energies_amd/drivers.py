@@ -70,8 +70,12 @@ class DeviceGoldenSearch:
         for _ in range(4 + max(n_loop, 0) + 1):
             self._point(xt, pt)
         st = self.state.cpu().numpy()
+        extra = 0
         while st[11] < 7.0:  # not finished (rounding made the bracket shrink more slowly)
+            if extra >= 200:
+                raise RuntimeError("device golden-section search did not finish")
             self._point(xt, pt)
+            extra += 1
             st = self.state.cpu().numpy()
         self.searches += 1
         return float(st[8]), float(st[9]), int(st[12])

@@ -359,7 +359,8 @@ def newton(
             )
             it["ls_evals"] += n_rep
             it["ls_repaired"] = rep
-            if device_linesearch is not None and project_fn is None and ghost_update_fn is None:
+            if (device_linesearch is not None and project_fn is None and ghost_update_fn is None
+                    and np.isfinite(alpha_lo) and np.isfinite(alpha_hi) and linesearch_tol > 0.0):
                 alpha, val, n_ev = device_linesearch(x_prev, p, alpha_lo, alpha_hi, linesearch_tol)
                 it["ls_evals"] += n_ev
                 x_trial.waxpy(alpha, p, x_prev)  # the accepted point, as energy_at(alpha) leaves it

@@ -20,4 +20,5 @@ PINNING = {
     "assembly": "brute-force dense double loop and scipy COO->CSR duplicate summation",
     "minimizers": "quadratic/1-D known answers; golden-section exactness on parabolas",
     "davis_b": "reference golden values tests/test_slope_stability_jax.py:20-24",
+    "c_assembly (fne_oracle.c)": "equal to assembly.assemble (energy, gradient, CSR Hessian to 1e-13) for p-Laplace and GL",
 }
