@@ -370,8 +370,18 @@ __global__ __launch_bounds__(kBlock) void mc_elem_kernel(fne_plan_desc d, int mo
   }
 }
 
-// Row gather.  A group of 16 lanes owns kMcRowBatch consecutive CSR rows; lane c handles the
-// element columns c and c + 16.  inc[it] = element * 32 + local row lists the incidences of
+template <typename T> struct Pair;
+template <> struct Pair<float> {
+  typedef float2 type;
+  static __device__ __forceinline__ float2 zero() { return make_float2(0.f, 0.f); }
+};
+template <> struct Pair<double> {
+  typedef double2 type;
+  static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
+};
+
+// Row gather.  A group of 16 lanes owns kMcRowBatch consecutive CSR rows; lane c < NL/2 handles
+// the element columns 2c and 2c+1 (paired loads).  inc[it] = element * 32 + local row lists the incidences of
 // every row (rows consecutive, elements ascending); pos[it*32 + c] is the position of the
 // element's local column c inside the row (255 = constrained).  The incidences of the whole
 // batch are read in one load (lane j holds item j), up to 8 items' element rows and positions
@@ -388,7 +398,6 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
   constexpr int kRB = kMcRowBatch;
   constexpr int kRowsBlk = (kBlock / kL) * kRB;
   constexpr int kPf = 8;
-  constexpr int kCols = (NL + kL - 1) / kL;  // element columns per lane
   const int cap = (d.max_row_len + 1) & ~1;
   const int grp = threadIdx.x / kL;
   const int c = threadIdx.x & (kL - 1);
@@ -430,33 +439,32 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
       }
     }
     if (WH) {
+      // lane c < NL/2 carries the element columns 2c and 2c+1: one paired value load and one
+      // 16-bit position load per item
+      typedef typename Pair<HT>::type HT2;
+      constexpr int kHalf = NL / 2;
       for (int j0 = 0; j0 < nb; j0 += kPf) {
-        HT v[kPf][kCols];
-        unsigned char pp[kPf][kCols];
+        HT2 v[kPf];
+        unsigned int pp[kPf];
         int sl[kPf];
 #pragma unroll
         for (int jj = 0; jj < kPf; ++jj) {
           const int j = j0 + jj;
           const unsigned int item = __shfl(mine, j < nb ? j : 0, kL);
           sl[jj] = __shfl(myslot, j < nb ? j : 0, kL);
-#pragma unroll
-          for (int q = 0; q < kCols; ++q) {
-            const int col = c + q * kL;
-            v[jj][q] = HT(0);
-            pp[jj][q] = kPosNone;
-            if (j < nb && col < NL) {
-              pp[jj][q] = pos[size_t(base + j) * 32 + col];
-              v[jj][q] = hbuf[size_t(item >> 5) * (NL * NL) + (item & 31u) * NL + col];
-            }
+          v[jj] = Pair<HT>::zero();
+          pp[jj] = 0xFFFFu;
+          if (j < nb && c < kHalf) {
+            pp[jj] = *reinterpret_cast<const unsigned short*>(pos + size_t(base + j) * 32 + 2 * c);
+            v[jj] = *reinterpret_cast<const HT2*>(hbuf + size_t(item >> 5) * (NL * NL) + (item & 31u) * NL + 2 * c);
           }
         }
 #pragma unroll
         for (int jj = 0; jj < kPf; ++jj) {
-#pragma unroll
-          for (int q = 0; q < kCols; ++q) {
-            if (pp[jj][q] != kPosNone) sRows[sl[jj] * cap + pp[jj][q]] += v[jj][q];
-            __builtin_amdgcn_wave_barrier();
-          }
+          const unsigned int p0 = pp[jj] & 0xFFu, p1 = pp[jj] >> 8;
+          if (p0 != kPosNone) sRows[sl[jj] * cap + p0] += v[jj].x;
+          if (p1 != kPosNone) sRows[sl[jj] * cap + p1] += v[jj].y;
+          __builtin_amdgcn_wave_barrier();
         }
       }
     }
