@@ -392,9 +392,9 @@ class SmoothedAggregationPC:
     def _lanczos_lambda_max(self, lvl: int, A0, dinv: torch.Tensor, v0: torch.Tensor) -> float:
         """Largest eigenvalue of D^-1 A (similar to S = D^-1/2 A D^-1/2) from LANCZOS_STEPS
         Lanczos steps on S from the fixed start vector v0; the recurrence stays on the device and
-        one transfer brings the tridiagonal coefficients to the host.  A Krylov estimate: the
-        power iteration it replaces could stay 20-30 % low after a few steps once plasticity
-        changed the spectrum, which made the smoother (and the V-cycle) indefinite."""
+        one transfer brings the tridiagonal coefficients to the host.  A Krylov estimate: a
+        few warm-started power iterations, used before, under-estimated rho once plasticity
+        changed the spectrum, and the smoother (and so the V-cycle) became indefinite."""
         s = torch.sqrt(dinv)
         mult = A0.mult_tensor if lvl == 0 else self.levels[lvl - 1]["Ac64"].mult
         v = v0 / torch.linalg.vector_norm(v0)
