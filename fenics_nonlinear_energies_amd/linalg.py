@@ -428,7 +428,8 @@ class KSP:
                 self.reason = KSP_DIVERGED_ITS
                 return
 
-    # Steihaug-Toint truncated CG inside radius (Euclidean norm on the unknown) ------------
+    # Steihaug-Toint truncated CG inside radius (Euclidean norm on the unknown; PETSc KSPSTCG's
+    # default direction type), stopping rules of KSPCG on the preconditioned residual --------
     def _solve_stcg(self, b: torch.Tensor, x: torch.Tensor) -> None:
         A = self.A
         delta = float("inf") if self.radius is None else float(self.radius)
@@ -445,16 +446,25 @@ class KSP:
         if self.reason:
             return
         rz = self._dot(r, z)
+        if rz < 0.0:
+            self.reason = KSP_DIVERGED_INDEFINITE_PC
+            return
         p = z.clone()
         xx = 0.0  # ||x||^2
         for i in range(self.max_it):
             A.mult_tensor(p, w)
             kappa = self._dot(p, w)
+            if not math.isfinite(kappa):
+                self.its = i + 1
+                self.reason = KSP_DIVERGED_NANORINF
+                return
             xp = self._dot(x, p)
             pp = self._dot(p, p)
-            if kappa <= 0.0 or not math.isfinite(kappa):
-                tau = (-xp + math.sqrt(max(xp * xp + pp * (delta * delta - xx), 0.0))) / pp
-                x.add_(p, alpha=tau)
+            if kappa <= 0.0:
+                # negative curvature: follow p to the boundary (no boundary: keep the iterate)
+                if math.isfinite(delta):
+                    tau = (-xp + math.sqrt(max(xp * xp + pp * (delta * delta - xx), 0.0))) / pp
+                    x.add_(p, alpha=tau)
                 self.its = i + 1
                 self.reason = KSP_CONVERGED_CG_NEG_CURVE
                 return

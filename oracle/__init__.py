@@ -19,6 +19,7 @@ PINNING = {
     "elements.mohr_coulomb": "FD + torch autodiff twin + elastic-branch B^T C B closed form + yield-surface invariant",
     "assembly": "brute-force dense double loop and scipy COO->CSR duplicate summation",
     "minimizers": "quadratic/1-D known answers; golden-section exactness on parabolas",
+    "solvers.cg/gmres/stcg/golden_section": "exact solves, k-distinct-eigenvalue termination, steepest-descent first step, Steihaug's monotone path, boundary and negative-curvature stops, golden-section evaluation count (tests/test_oracle_solvers.py)",
     "davis_b": "reference golden values tests/test_slope_stability_jax.py:20-24",
     "c_assembly (fne_oracle.c)": "equal to assembly.assemble (energy, gradient, CSR Hessian to 1e-13) for p-Laplace and GL",
 }

@@ -4,6 +4,7 @@
   diagonal entries replaced by 1, convergence on the preconditioned residual norm
   ||B r_k|| <= rtol ||B r_0||, indefinite-curvature stop, iteration cap).
 * ``gmres``: PETSc KSPGMRES(restart) with left preconditioning, modified Gram-Schmidt.
+* ``stcg``: PETSc KSPSTCG, the trust-region (Steihaug-Toint) CG of the trust-region Newton.
 * ``newton_golden``: the line-search branch of core/petsc/minimizers.py:181-1038
   (golden-section search on [a, b] after non-finite interval repair, convergence on
   |dE| < tolf and small step, then a gradient check when require_all_convergence).
@@ -118,6 +119,65 @@ def gmres(A, b, rtol=1e-3, max_it=10000, restart=30, atol=1e-50, inv_diag=None):
             return x, its, done
         if its >= max_it:
             return x, its, -3
+
+
+def stcg(A, b, radius=math.inf, rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, inv_diag=None):
+    """Steihaug-Toint truncated CG: approximately minimise q(s) = -b.s + s.A s / 2 over ||s|| <= radius.
+
+    The trust-region Krylov solve the reference requests with ksp_type "stcg" and a radius set
+    through KSPCGSetRadius (core/petsc/trust_ksp.py:55-72, core/petsc/scalar_problem_driver.py:
+    144-170).  The algorithm is PETSc's KSPSTCG (PETSc 3.2x, absent here; Steihaug 1983, Toint
+    1981): preconditioned CG from s = 0; the constraint on the Euclidean norm of s (PETSc's
+    default "unpreconditioned direction"); on curvature p.Ap <= 0 (KSP_CONVERGED_CG_NEG_CURVE,
+    5) or on a step that would leave the ball (KSP_CONVERGED_STEP_LENGTH, 6) the iterate moves
+    along p to the boundary and stops.  Convergence on the preconditioned residual norm as in
+    ``cg``.  With an infinite radius a negative curvature stop keeps the current iterate.
+    Returns (s, its, reason).
+    """
+    inv_diag = jacobi_inverse(A) if inv_diag is None else inv_diag
+    delta = float(radius)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = inv_diag * r
+    r0 = math.sqrt(float(z @ z))
+    if r0 <= max(rtol * r0, atol):
+        return x, 0, 2 if r0 > atol else 3
+    rz = float(r @ z)
+    if rz < 0.0:
+        return x, 0, -8
+    p = z.copy()
+    xx = 0.0
+    for i in range(max_it):
+        w = A @ p
+        kappa = float(p @ w)
+        if not math.isfinite(kappa):
+            return x, i + 1, -9
+        xp, pp = float(x @ p), float(p @ p)
+
+        def to_boundary():
+            # the positive root tau of ||x + tau p||^2 = delta^2
+            return (-xp + math.sqrt(max(xp * xp + pp * (delta * delta - xx), 0.0))) / pp
+
+        if kappa <= 0.0:
+            if math.isfinite(delta):
+                x = x + to_boundary() * p
+            return x, i + 1, 5
+        alpha = rz / kappa
+        if xx + 2.0 * alpha * xp + alpha * alpha * pp >= delta * delta:
+            return x + to_boundary() * p, i + 1, 6
+        x = x + alpha * p
+        xx = float(x @ x)
+        r = r - alpha * w
+        z = inv_diag * r
+        dp = math.sqrt(float(z @ z))
+        if dp <= max(rtol * r0, atol):
+            return x, i + 1, 2 if dp > atol else 3
+        if dp >= dtol * r0:
+            return x, i + 1, -4
+        rz_new = float(r @ z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_it, -3
 
 
 def golden_section(f, a, b, tol):

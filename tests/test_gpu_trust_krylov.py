@@ -1,0 +1,152 @@
+"""Device GMRES and Steihaug-Toint CG against the oracle, and the trust-region Newton driver.
+
+* GMRES(restart) over the device CSR operator: same iteration count and reason as the
+  oracle's KSPGMRES restatement, solution to 1e-10 relative, on an assembled p-Laplace
+  Hessian and on a nonsymmetric sparse matrix.
+* STCG (the trust-region subproblem solver, ksp_type "stcg"): same iteration count and
+  reason as the oracle for no radius, a radius below the first step, a radius the path
+  crosses, and negative curvature (the Ginzburg-Landau Hessian at u = 0 is indefinite);
+  solutions to 1e-10 relative; boundary stops end on the sphere.
+* trust-region Newton on the reference p-Laplace L5 mesh, with STCG subproblems and with the
+  reference's 1-D / 2-D model subproblems: both converge to the energy of line-search Newton
+  and to the reference's published value (p-Laplace is strictly convex: one minimiser).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from fenics_nonlinear_energies_amd.drivers import solve_newton
+from fenics_nonlinear_energies_amd.linalg import KSP, DeviceCSR
+from fenics_nonlinear_energies_amd.problems.ginzburg_landau import GLReorderedElementAssembler, build_gl_problem
+from fenics_nonlinear_energies_amd.problems.plaplace import (
+    PLaplaceReorderedElementAssembler,
+    build_plaplace_problem,
+    reference_random_init,
+)
+from fenics_nonlinear_energies_amd.vec import DeviceVec
+from oracle import solvers
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+
+
+def _plaplace_hessian(n=48, seed=0):
+    params, adj, _ = build_plaplace_problem(n)
+    asm = PLaplaceReorderedElementAssembler(params, None, adj, ksp_type="cg")
+    u = np.random.default_rng(seed).random(params["freedofs"].size)
+    asm.assemble_hessian(asm.create_vec(u[asm.layout.perm]))
+    return asm
+
+
+def _device_csr(M: sp.csr_matrix) -> DeviceCSR:
+    M = M.tocsr()
+    M.sort_indices()
+    A = DeviceCSR(M.indptr, M.indices, M.shape[0], precision="fp64")
+    A.data.copy_(torch.as_tensor(M.data, dtype=torch.float64))
+    return A
+
+
+def _vec(a: np.ndarray) -> DeviceVec:
+    return DeviceVec(torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda"))
+
+
+def _ksp_solve(A: DeviceCSR, b: np.ndarray, ksp_type: str, rtol: float, radius=None, restart=30, max_it=10000):
+    ksp = KSP(ksp_type=ksp_type, pc_type="jacobi", rtol=rtol, max_it=max_it, restart=restart)
+    ksp.setOperators(A)
+    ksp.setUp()
+    if radius is not None:
+        ksp.setCGRadius(radius)
+    bv, xv = _vec(b), _vec(np.zeros_like(b))
+    ksp.solve(bv, xv)
+    return xv.array.copy(), ksp.getIterationNumber(), ksp.getConvergedReason()
+
+
+# Restarted runs whose residual creeps down for hundreds of iterations (GMRES(7) at 1e-8 here)
+# can cross rtol one iteration apart under different rounding; these cases converge steadily.
+@pytest.mark.parametrize("rtol,restart", [(1e-3, 30), (1e-10, 30)])
+def test_device_gmres_matches_oracle_on_hessian(require_gpu, rtol, restart):
+    asm = _plaplace_hessian()
+    A = asm.A.to_scipy()
+    b = np.random.default_rng(1).standard_normal(asm.layout.n_free)
+    x, its, reason = _ksp_solve(asm.A, b, "gmres", rtol, restart=restart)
+    xo, its_o, reason_o = solvers.gmres(A, b, rtol=rtol, restart=restart)
+    assert (its, reason) == (its_o, reason_o)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    asm.cleanup()
+
+
+@pytest.mark.parametrize("restart", [20, 3])
+def test_device_gmres_matches_oracle_nonsymmetric(require_gpu, restart):
+    rng = np.random.default_rng(2)
+    n = 3000
+    M = sp.random(n, n, density=4.0 / n, random_state=3, format="csr") + sp.diags(4.0 + rng.random(n))
+    M = sp.csr_matrix(M)
+    b = rng.standard_normal(n)
+    x, its, reason = _ksp_solve(_device_csr(M), b, "gmres", 1e-9, restart=restart)
+    xo, its_o, reason_o = solvers.gmres(M, b, rtol=1e-9, restart=restart)
+    assert (its, reason) == (its_o, reason_o) and reason == 2
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_device_stcg_matches_oracle(require_gpu):
+    asm = _plaplace_hessian()
+    A = asm.A.to_scipy()
+    b = np.random.default_rng(4).standard_normal(asm.layout.n_free)
+    # unconstrained path: its length fixes the radii below
+    x_inf, its_inf, reason_inf = _ksp_solve(asm.A, b, "stcg", 1e-6)
+    xo, its_o, reason_o = solvers.stcg(A, b, rtol=1e-6)
+    assert (its_inf, reason_inf) == (its_o, reason_o) and reason_o == 2
+    assert np.abs(x_inf - xo).max() <= 1e-10 * np.abs(xo).max()
+    full = float(np.linalg.norm(xo))
+    for frac in (1e-4, 0.3, 0.9):
+        delta = frac * full
+        x, its, reason = _ksp_solve(asm.A, b, "stcg", 1e-6, radius=delta)
+        xo, its_o, reason_o = solvers.stcg(A, b, radius=delta, rtol=1e-6)
+        assert (its, reason) == (its_o, reason_o) and reason == 6
+        assert abs(np.linalg.norm(x) - delta) <= 1e-10 * delta
+        assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    asm.cleanup()
+
+
+def test_device_stcg_negative_curvature_gl(require_gpu):
+    """GL Hessian at u = 0 is eps K - M (the potential's curvature 3u^2 - 1 = -1): indefinite.
+
+    A constant right-hand side meets negative curvature on the first direction; a random one
+    on the 7th (p.Ap / p.p = -1.2e-4 against 4e-2 at the start, far from rounding)."""
+    params, adj = build_gl_problem(40)
+    asm = GLReorderedElementAssembler(params, None, adj, ksp_type="cg")
+    asm.assemble_hessian(asm.create_vec(np.zeros(asm.layout.n_free)))
+    A = asm.A.to_scipy()
+    assert np.linalg.eigvalsh(A.toarray())[0] < 0.0
+    n = asm.layout.n_free
+    for b, delta, its_expected in ((np.ones(n), 0.5, 1), (np.random.default_rng(5).standard_normal(n), 2e4, 7)):
+        x, its, reason = _ksp_solve(asm.A, b, "stcg", 1e-8, radius=delta)
+        xo, its_o, reason_o = solvers.stcg(A, b, radius=delta, rtol=1e-8)
+        assert (its_o, reason_o) == (its_expected, 5)
+        assert (its, reason) == (its_o, reason_o)
+        assert abs(np.linalg.norm(x) - delta) <= 1e-10 * delta
+        assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+    asm.cleanup()
+
+
+@pytest.mark.parametrize("ksp_type", ["stcg", "cg"])
+def test_trust_region_newton_plaplace_l5(require_gpu, ksp_type):
+    d, adj = fixtures.plaplace(5)
+    u0 = reference_random_init(d["freedofs"].size)
+    pub = [r for r in fixtures.published_energies()["plaplace"] if r["level"] == 5][0]["jax_petsc_element"]
+    asm = PLaplaceReorderedElementAssembler(d, None, adj, ksp_type=ksp_type, pc_type="jacobi", ksp_rtol=1e-3)
+    x = asm.create_vec(u0[asm.layout.perm])
+    res = solve_newton(asm, x, tolf=1e-5, tolg=1e-3, ksp_rtol=1e-3, use_trust_region=True, trust_radius_init=1.0,
+                       maxit=200)
+    print(ksp_type, res["message"], res["nit"], res["fun"], flush=True)
+    assert asm.ksp.ksp_type == ksp_type  # stcg: radius-constrained subproblems; cg: 1-D / 2-D models
+    assert "converged" in res["message"].lower(), res["message"]
+    assert abs(res["fun"] - pub) <= 1e-8 * abs(pub)
+    g = x.duplicate()
+    asm.gradient_fn(x, g)
+    assert math.isfinite(g.norm()) and g.norm() < 1e-2
+    asm.cleanup()
