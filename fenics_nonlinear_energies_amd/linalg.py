@@ -5,8 +5,10 @@
 * ``KSP`` reproduces the PETSc KSP semantics the reference relies on -- zero initial guess,
   left preconditioning, convergence on the preconditioned residual norm
   ``||B r_k|| <= max(rtol*||B r_0||, abstol)``, divergence at ``dtol``, iteration cap -- for
-  ``cg`` (KSPCG), ``gmres`` (KSPGMRES, restart 30, modified Gram-Schmidt) and ``stcg``
-  (Steihaug-Toint CG with a radius, as set by KSPCGSetRadius in core/petsc/trust_ksp.py).
+  ``cg`` (KSPCG) and ``gmres`` (KSPGMRES, restart 30, modified Gram-Schmidt), and ``stcg``
+  (Steihaug-Toint CG with a radius, as set by KSPCGSetRadius in core/petsc/trust_ksp.py),
+  which tests the unpreconditioned residual norm ``||r_k||`` instead, as PETSc's KSPSTCG does
+  by default.
   The preconditioner is point Jacobi (``jacobi``; zero diagonal entries are replaced by 1
   as PETSc's PCJACOBI does), ``none``, or ``gamg`` (alias ``hypre``): the smoothed-aggregation
   AMG V-cycle of ``amg.py`` with the problem's near-nullspace.
@@ -428,8 +430,11 @@ class KSP:
                 self.reason = KSP_DIVERGED_ITS
                 return
 
-    # Steihaug-Toint truncated CG inside radius (Euclidean norm on the unknown; PETSc KSPSTCG's
-    # default direction type), stopping rules of KSPCG on the preconditioned residual --------
+    # Steihaug-Toint truncated CG inside radius: PETSc KSPSTCG as the trust-region Newton uses
+    # it (core/petsc/trust_ksp.py:55-72), restated for the serial path in
+    # core/serial/sparse_solvers.py:20-93.  Euclidean norm on the step (PETSc's default
+    # direction type), convergence on the unpreconditioned residual norm ||r|| (PETSc STCG's
+    # default norm, the serial restatement's test), boundary / negative-curvature stops ----
     def _solve_stcg(self, b: torch.Tensor, x: torch.Tensor) -> None:
         A = self.A
         delta = float("inf") if self.radius is None else float(self.radius)
@@ -438,7 +443,7 @@ class KSP:
         z = torch.empty_like(b)
         w = torch.empty_like(b)
         self._pc(r, z)
-        dp = math.sqrt(max(self._dot(z, z), 0.0))
+        dp = math.sqrt(max(self._dot(r, r), 0.0))
         rnorm0 = dp
         self.its = 0
         self.rnorm = dp
@@ -480,7 +485,7 @@ class KSP:
             xx = xx_new
             r.add_(w, alpha=-alpha)
             self._pc(r, z)
-            dp = math.sqrt(max(self._dot(z, z), 0.0))
+            dp = math.sqrt(max(self._dot(r, r), 0.0))
             self.its = i + 1
             self.rnorm = dp
             self.reason = self._converged(i + 1, dp, rnorm0)

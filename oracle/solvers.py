@@ -126,20 +126,22 @@ def stcg(A, b, radius=math.inf, rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, i
 
     The trust-region Krylov solve the reference requests with ksp_type "stcg" and a radius set
     through KSPCGSetRadius (core/petsc/trust_ksp.py:55-72, core/petsc/scalar_problem_driver.py:
-    144-170).  The algorithm is PETSc's KSPSTCG (PETSc 3.2x, absent here; Steihaug 1983, Toint
-    1981): preconditioned CG from s = 0; the constraint on the Euclidean norm of s (PETSc's
-    default "unpreconditioned direction"); on curvature p.Ap <= 0 (KSP_CONVERGED_CG_NEG_CURVE,
-    5) or on a step that would leave the ball (KSP_CONVERGED_STEP_LENGTH, 6) the iterate moves
-    along p to the boundary and stops.  Convergence on the preconditioned residual norm as in
-    ``cg``.  With an infinite radius a negative curvature stop keeps the current iterate.
-    Returns (s, its, reason).
+    143-170).  The algorithm is PETSc's KSPSTCG (PETSc 3.2x, absent here; Steihaug 1983, Toint
+    1981), which the reference restates for its serial path in
+    core/serial/sparse_solvers.py:20-93: preconditioned CG from s = 0 with the constraint on the
+    Euclidean norm of s; on curvature p.Ap <= 0 (KSP_CONVERGED_CG_NEG_CURVE, 5) or on a step
+    that would leave the ball (KSP_CONVERGED_STEP_LENGTH, 6) the iterate moves along p to the
+    boundary and stops; convergence on the unpreconditioned residual norm
+    ||r_k|| <= max(rtol ||b||, atol) (the serial restatement's test and PETSc STCG's default
+    norm), divergence at dtol ||b||.  With an infinite radius a negative curvature stop keeps
+    the current iterate.  Returns (s, its, reason).
     """
     inv_diag = jacobi_inverse(A) if inv_diag is None else inv_diag
     delta = float(radius)
     x = np.zeros_like(b)
     r = b.copy()
     z = inv_diag * r
-    r0 = math.sqrt(float(z @ z))
+    r0 = math.sqrt(float(r @ r))
     if r0 <= max(rtol * r0, atol):
         return x, 0, 2 if r0 > atol else 3
     rz = float(r @ z)
@@ -169,7 +171,7 @@ def stcg(A, b, radius=math.inf, rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, i
         xx = float(x @ x)
         r = r - alpha * w
         z = inv_diag * r
-        dp = math.sqrt(float(z @ z))
+        dp = math.sqrt(float(r @ r))
         if dp <= max(rtol * r0, atol):
             return x, i + 1, 2 if dp > atol else 3
         if dp >= dtol * r0:

@@ -5,7 +5,8 @@
   step; a sign change of p.Ap is reported as an indefinite matrix.
 * gmres: the solution of a nonsymmetric system, also with restarts; at most k iterations for
   a diagonalisable matrix with k distinct eigenvalues; the returned residual meets rtol.
-* stcg: with no radius it is cg; a radius below the first step gives the scaled first
+* stcg: with no radius (and no preconditioner) it is cg; it stops on the unpreconditioned
+  residual; a radius below the first step gives the scaled first
   direction; the iterates grow in norm while the quadratic model falls (Steihaug 1983,
   Thm 2.1), so a radius between two iterates gives a boundary point better than every
   interior one; negative curvature ends on the boundary along p.
@@ -107,12 +108,26 @@ def test_gmres_k_distinct_eigenvalues_k_iterations():
 
 
 def test_stcg_without_radius_is_cg():
+    # without a preconditioner both test the same residual norm
     A, b = _spd(40, 1e3, 6)
+    one = np.ones(40)
     for rtol in (1e-3, 1e-10):
-        xc, itc, rc = solvers.cg(A, b, rtol=rtol)
-        xs, its, rs = solvers.stcg(A, b, radius=math.inf, rtol=rtol)
+        xc, itc, rc = solvers.cg(A, b, rtol=rtol, inv_diag=one)
+        xs, its, rs = solvers.stcg(A, b, radius=math.inf, rtol=rtol, inv_diag=one)
         assert (its, rs) == (itc, rc)
         assert np.abs(xs - xc).max() <= 1e-12 * np.abs(xc).max()
+
+
+def test_stcg_stops_on_the_unpreconditioned_residual():
+    # Jacobi with a badly scaled diagonal separates ||r|| from ||D^-1 r||
+    A, b = _spd(40, 1e2, 9)
+    s = np.geomspace(1.0, 1e3, 40)
+    A = sp.csr_matrix(sp.diags(s) @ A @ sp.diags(s))
+    x, its, reason = solvers.stcg(A, b, rtol=1e-6)
+    assert reason == 2 and its >= 2
+    assert np.linalg.norm(b - A @ x) <= 1e-6 * np.linalg.norm(b) * (1 + 1e-8)
+    x1, _, _ = solvers.stcg(A, b, rtol=1e-6, max_it=its - 1)
+    assert np.linalg.norm(b - A @ x1) > 1e-6 * np.linalg.norm(b)
 
 
 def test_stcg_small_radius_is_scaled_first_direction():
