@@ -10,6 +10,14 @@
 // ``check_every`` iterations.  Iterations enqueued after the stop are no-ops, so the result
 // and the iteration count are exactly those of the iteration that met the test.  All dot
 // products are compensated FP64 sums in a fixed order: the solve is bitwise reproducible.
+//
+// Trust mode (fne_pstcg, fne_pstcg_graph): PETSc's KSPSTCG, the Steihaug-Toint CG the
+// trust-region Newton solves its subproblems with (core/petsc/trust_ksp.py:55-72,
+// core/petsc/scalar_problem_driver.py:143-170; restated for the serial path in
+// core/serial/sparse_solvers.py:20-93).  The same four launches carry two more dot products
+// (p.p and x.p, fused into the SpMV pass) and ||x||^2 by recurrence; a step that would leave
+// the ball ||x|| <= radius, or curvature p.Ap <= 0, moves x along p to the sphere and ends
+// the solve (reasons 6 and 5); the stopping test uses the unpreconditioned residual ||r||.
 #include "fne_common.h"
 
 #include <algorithm>
@@ -29,10 +37,13 @@ enum : int {
   kDone,
   kIts,
   kReason,
+  kXX,     // trust mode: ||x||^2
+  kFinal,  // trust mode: the pending update is the last one (boundary step)
   kStateLen = 16
 };
 
-constexpr int kConvRtol = 2, kConvAtol = 3, kDivDtol = -4, kDivIndefPc = -8, kDivNan = -9, kDivIndefMat = -10;
+constexpr int kConvRtol = 2, kConvAtol = 3, kConvNegCurve = 5, kConvStepLength = 6, kDivDtol = -4, kDivIndefPc = -8,
+              kDivNan = -9, kDivIndefMat = -10;
 
 struct CgParams {
   int n;
@@ -40,6 +51,8 @@ struct CgParams {
   const int32_t* indices;
   const double* inv_diag;  // null -> identity
   double rtol, atol, dtol;
+  int trust;      // Steihaug-Toint mode
+  double radius;  // trust mode: radius of the ball (inf: none)
 };
 
 __device__ __forceinline__ int cg_converged(int it, double rn, double rn0, const CgParams& P) {
@@ -70,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(CgParams P, const doubl
     z[i] = zi;
     x[i] = 0.0;
     p0[i] = 0.0;
-    acc_add(s1, c1, zi * zi);
+    acc_add(s1, c1, P.trust ? ri * ri : zi * zi);
     acc_add(s2, c2, zi * ri);
   }
   block_reduce_pair(s1, c1);
@@ -97,6 +110,8 @@ __global__ __launch_bounds__(1024) void cg_init_finalize(CgParams P, const doubl
   st[kRatio] = 0.0;
   st[kDone] = 0.0;
   st[kReason] = 0.0;
+  st[kXX] = 0.0;
+  st[kFinal] = 0.0;
   int reason = cg_converged(0, dp, dp, P);
   const double beta = zr + czr;
   if (!reason && beta < 0.0) reason = kDivIndefPc;
@@ -109,11 +124,13 @@ __global__ __launch_bounds__(1024) void cg_init_finalize(CgParams P, const doubl
 }
 
 // p_new = z + ratio p_old; w = A p_new (on the fly from z and p_old); partials of p_new.w
+// (trust mode also p_new.p_new and x.p_new, at part + gridDim.x and part + 2 gridDim.x)
 template <typename VT, int LPR>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* __restrict__ vals,
                                                          const double* __restrict__ z,
                                                          const double* __restrict__ p_old,
                                                          double* __restrict__ p_new, double* __restrict__ w,
+                                                         const double* __restrict__ x,
                                                          const double* __restrict__ st,
                                                          double2* __restrict__ part) {
   if (st[kDone] != 0.0) return;
@@ -131,22 +148,80 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* _
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-  double s = 0.0, cc = 0.0;
+  double s = 0.0, cc = 0.0, s2 = 0.0, c2 = 0.0, s3 = 0.0, c3 = 0.0;
   if (row < P.n && lane == 0) {
     const double pn = z[row] + ratio * p_old[row];
     p_new[row] = pn;
     w[row] = acc;
     acc_add(s, cc, pn * acc);
+    if (P.trust) {
+      acc_add(s2, c2, pn * pn);
+      acc_add(s3, c3, x[row] * pn);
+    }
   }
   block_reduce_pair(s, cc);
-  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(s, cc);
+  if (P.trust) {  // uniform over the grid
+    __syncthreads();
+    block_reduce_pair(s2, c2);
+    __syncthreads();
+    block_reduce_pair(s3, c3);
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = make_double2(s, cc);
+    if (P.trust) {
+      part[gridDim.x + blockIdx.x] = make_double2(s2, c2);
+      part[2 * gridDim.x + blockIdx.x] = make_double2(s3, c3);
+    }
+  }
 }
 
-__global__ __launch_bounds__(1024) void cg_alpha_finalize(const double2* __restrict__ part, int nb,
+__global__ __launch_bounds__(1024) void cg_alpha_finalize(CgParams P, const double2* __restrict__ part, int nb,
                                                           double* __restrict__ st) {
   if (st[kDone] != 0.0) return;
   double s, c;
   merge_partials(part, nb, s, c);
+  if (P.trust) {
+    double pp, cpp, xp, cxp;
+    __syncthreads();
+    merge_partials(part + nb, nb, pp, cpp);
+    __syncthreads();
+    merge_partials(part + 2 * nb, nb, xp, cxp);
+    if (threadIdx.x != 0) return;
+    const double kappa = s + c;
+    pp += cpp;
+    xp += cxp;
+    if (!isfinite(kappa)) {
+      st[kIts] += 1.0;
+      st[kReason] = kDivNan;
+      st[kDone] = 1.0;
+      return;
+    }
+    const double delta = P.radius, xx = st[kXX];
+    // positive root tau of ||x + tau p||^2 = delta^2
+    const double tau = (-xp + sqrt(fmax(xp * xp + pp * (delta * delta - xx), 0.0))) / pp;
+    if (kappa <= 0.0) {
+      st[kReason] = kConvNegCurve;
+      if (isfinite(delta)) {
+        st[kAlpha] = tau;
+        st[kFinal] = 1.0;
+      } else {  // no boundary to move to: keep the iterate
+        st[kIts] += 1.0;
+        st[kDone] = 1.0;
+      }
+      return;
+    }
+    const double alpha = st[kBeta] / kappa;
+    const double xx_new = xx + 2.0 * alpha * xp + alpha * alpha * pp;
+    if (xx_new >= delta * delta) {
+      st[kReason] = kConvStepLength;
+      st[kAlpha] = tau;
+      st[kFinal] = 1.0;
+      return;
+    }
+    st[kXX] = xx_new;
+    st[kAlpha] = alpha;
+    return;
+  }
   if (threadIdx.x != 0) return;
   const double dpi = s + c;
   const double old = st[kDpiOld];
@@ -167,15 +242,19 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgParams P, const dou
                                                            double2* __restrict__ part) {
   if (st[kDone] != 0.0) return;
   const double a = st[kAlpha];
-  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
   const int stride = gridDim.x * blockDim.x;
+  if (st[kFinal] != 0.0) {  // trust mode: the boundary step ends the solve
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += stride) x[i] += a * p[i];
+    return;
+  }
+  double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += stride) {
     x[i] += a * p[i];
     const double ri = r[i] - a * w[i];
     r[i] = ri;
     const double zi = P.inv_diag ? ri * P.inv_diag[i] : ri;
     z[i] = zi;
-    acc_add(s1, c1, zi * zi);
+    acc_add(s1, c1, P.trust ? ri * ri : zi * zi);
     acc_add(s2, c2, zi * ri);
   }
   block_reduce_pair(s1, c1);
@@ -190,12 +269,19 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgParams P, const dou
 __global__ __launch_bounds__(1024) void cg_beta_finalize(CgParams P, const double2* __restrict__ part, int nb,
                                                          double* __restrict__ st) {
   if (st[kDone] != 0.0) return;
+  if (st[kFinal] != 0.0) {  // trust mode: boundary step taken, reason already set
+    if (threadIdx.x == 0) {
+      st[kIts] += 1.0;
+      st[kDone] = 1.0;
+    }
+    return;
+  }
   double zz, czz, zr, czr;
   merge_partials(part, nb, zz, czz);
   __syncthreads();
   merge_partials(part + nb, nb, zr, czr);
   if (threadIdx.x != 0) return;
-  const double dp = sqrt(fmax(zz + czz, 0.0));
+  const double dp = sqrt(fmax(zz + czz, 0.0));  // ||z||, or ||r|| in trust mode
   const int its = int(st[kIts]) + 1;
   st[kIts] = its;
   st[kRnorm] = dp;
@@ -223,24 +309,25 @@ __global__ __launch_bounds__(kBlock) void cg_update_xr_kernel(int n, const doubl
                                                               double* __restrict__ r, const double* __restrict__ st) {
   if (st[kDone] != 0.0) return;
   const double a = st[kAlpha];
+  const bool last = st[kFinal] != 0.0;
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     x[i] += a * p[i];
-    r[i] -= a * w[i];
+    if (!last) r[i] -= a * w[i];
   }
 }
 
-// partials of z.z and z.r (z = M r from the preconditioner)
-__global__ __launch_bounds__(kBlock) void cg_dots_kernel(int n, const double* __restrict__ z,
+// partials of z.z (r.r in trust mode) and z.r (z = M r from the preconditioner)
+__global__ __launch_bounds__(kBlock) void cg_dots_kernel(int n, int trust, const double* __restrict__ z,
                                                          const double* __restrict__ r, const double* __restrict__ st,
                                                          double2* __restrict__ part) {
-  if (st != nullptr && st[kDone] != 0.0) return;
+  if (st != nullptr && (st[kDone] != 0.0 || st[kFinal] != 0.0)) return;
   double s1 = 0.0, c1 = 0.0, s2 = 0.0, c2 = 0.0;
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double zi = z[i];
-    acc_add(s1, c1, zi * zi);
-    acc_add(s2, c2, zi * r[i]);
+    const double zi = z[i], ri = r[i];
+    acc_add(s1, c1, trust ? ri * ri : zi * zi);
+    acc_add(s2, c2, zi * ri);
   }
   block_reduce_pair(s1, c1);
   __syncthreads();
@@ -262,21 +349,22 @@ int lanes_for(int n, int64_t nnz) {
 
 template <typename VT, int LPR>
 void launch_cg_spmv(const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
-                    double* w, const double* st, double2* part, int nbs, hipStream_t s) {
-  hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, st, part);
+                    double* w, const double* x, const double* st, double2* part, int nbs, hipStream_t s) {
+  hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, x, st,
+                     part);
 }
 
 template <typename VT>
 void cg_spmv(int lpr, const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
-             double* w, const double* st, double2* part, int nbs, hipStream_t s) {
+             double* w, const double* x, const double* st, double2* part, int nbs, hipStream_t s) {
   switch (lpr) {
-    case 1: launch_cg_spmv<VT, 1>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    case 2: launch_cg_spmv<VT, 2>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    case 4: launch_cg_spmv<VT, 4>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    case 8: launch_cg_spmv<VT, 8>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    case 16: launch_cg_spmv<VT, 16>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    case 32: launch_cg_spmv<VT, 32>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
-    default: launch_cg_spmv<VT, 64>(P, vals, z, p_old, p_new, w, st, part, nbs, s); break;
+    case 1: launch_cg_spmv<VT, 1>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 2: launch_cg_spmv<VT, 2>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 4: launch_cg_spmv<VT, 4>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 8: launch_cg_spmv<VT, 8>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 16: launch_cg_spmv<VT, 16>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 32: launch_cg_spmv<VT, 32>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    default: launch_cg_spmv<VT, 64>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
   }
 }
 
@@ -289,17 +377,21 @@ extern "C" {
 int64_t fne_pcg_workspace_bytes(int32_t n, int64_t nnz) {
   if (n < 0 || nnz < 0) return FNE_EINVAL;
   const int lpr = lanes_for(n, nnz);
-  int64_t nparts = spmv_blocks(n, lpr);
+  int64_t nparts = 3 * spmv_blocks(n, lpr);  // p.w, and in trust mode p.p and x.p
   if (nparts < 2 * kVecBlocks) nparts = 2 * kVecBlocks;
   return int64_t(sizeof(double)) * (5 * int64_t(n) + kStateLen) + int64_t(sizeof(double2)) * nparts;
 }
 
-int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
-            const double* inv_diag, const double* b, double* x, double rtol, double atol, double dtol,
-            int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
-            int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
-  if (n < 0 || !indptr || !indices || !vals || !b || !x || !work || max_it < 0) {
-    fne_set_error("fne_pcg: invalid arguments");
+}  // extern "C"
+
+namespace {
+
+int pcg_run(const char* name, int trust, double radius, int32_t n, const int32_t* indptr, const int32_t* indices,
+            const void* vals, int32_t precision, const double* inv_diag, const double* b, double* x, double rtol,
+            double atol, double dtol, int32_t max_it, int32_t check_every, void* work, int64_t work_bytes,
+            int32_t* its_out, int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  if (n < 0 || !indptr || !indices || !vals || !b || !x || !work || max_it < 0 || (trust && !(radius >= 0.0))) {
+    fne_set_error(std::string(name) + ": invalid arguments");
     return FNE_EINVAL;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -307,14 +399,14 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
   {
     int32_t last = 0;
     if (n > 0 && hipMemcpyAsync(&last, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) {
-      fne_set_error("fne_pcg: reading indptr failed");
+      fne_set_error(std::string(name) + ": reading indptr failed");
       return FNE_EHIP;
     }
     FNE_HIP_CHECK(hipStreamSynchronize(s));
     nnz64 = last;
   }
   if (work_bytes < fne_pcg_workspace_bytes(n, nnz64)) {
-    fne_set_error("fne_pcg: workspace too small");
+    fne_set_error(std::string(name) + ": workspace too small");
     return FNE_EINVAL;
   }
   if (check_every < 1) check_every = 1;
@@ -327,7 +419,7 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
   double* pb = pa + n;
   double* st = pb + n;
   double2* part = reinterpret_cast<double2*>(st + kStateLen);
-  CgParams P{n, indptr, indices, inv_diag, rtol, atol, dtol};
+  CgParams P{n, indptr, indices, inv_diag, rtol, atol, dtol, trust, radius};
   const int nbv = n > 0 ? int(std::min<int64_t>(kVecBlocks, (int64_t(n) + kBlock - 1) / kBlock)) : 1;
   hipLaunchKernelGGL(cg_init_kernel, dim3(nbv), dim3(kBlock), 0, s, P, b, x, r, z, pa, part);
   hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
@@ -344,10 +436,10 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
       double* p_old = (it & 1) ? pb : pa;
       double* p_new = (it & 1) ? pa : pb;
       if (precision == FNE_FP32)
-        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, st, part, nbs, s);
+        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
       else
-        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, st, part, nbs, s);
-      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, part, nbs, st);
+        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, P, part, nbs, st);
       hipLaunchKernelGGL(cg_update_kernel, dim3(nbv), dim3(kBlock), 0, s, P, p_new, w, x, r, z, st, part);
       hipLaunchKernelGGL(cg_beta_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
     }
@@ -362,20 +454,18 @@ int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void
   return FNE_OK;
 }
 
-}  // extern "C"
-
-extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
-                             int32_t precision, void* pc_graph_exec, double* pc_in, const double* pc_out,
-                             const double* b, double* x, double rtol, double atol, double dtol, int32_t max_it,
-                             int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
-                             int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+int pcg_graph_run(const char* name, int trust, double radius, int32_t n, const int32_t* indptr,
+                  const int32_t* indices, const void* vals, int32_t precision, void* pc_graph_exec, double* pc_in,
+                  const double* pc_out, const double* b, double* x, double rtol, double atol, double dtol,
+                  int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+                  int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
   if (n < 0 || !indptr || !indices || !vals || !b || !x || !work || max_it < 0 || !pc_graph_exec || !pc_in ||
-      !pc_out || pc_in == pc_out) {
-    fne_set_error("fne_pcg_graph: invalid arguments");
+      !pc_out || pc_in == pc_out || (trust && !(radius >= 0.0))) {
+    fne_set_error(std::string(name) + ": invalid arguments");
     return FNE_EINVAL;
   }
   if (precision != FNE_FP32 && precision != FNE_FP64) {
-    fne_set_error("fne_pcg_graph: precision must be FNE_FP64 or FNE_FP32");
+    fne_set_error(std::string(name) + ": precision must be FNE_FP64 or FNE_FP32");
     return FNE_EINVAL;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -384,14 +474,14 @@ extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* in
   {
     int32_t last = 0;
     if (n > 0 && hipMemcpyAsync(&last, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) {
-      fne_set_error("fne_pcg_graph: reading indptr failed");
+      fne_set_error(std::string(name) + ": reading indptr failed");
       return FNE_EHIP;
     }
     FNE_HIP_CHECK(hipStreamSynchronize(s));
     nnz64 = last;
   }
   if (work_bytes < fne_pcg_workspace_bytes(n, nnz64)) {
-    fne_set_error("fne_pcg_graph: workspace too small");
+    fne_set_error(std::string(name) + ": workspace too small");
     return FNE_EINVAL;
   }
   if (check_every < 1) check_every = 1;
@@ -406,11 +496,11 @@ extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* in
   double* pb = pa + n;
   double* st = pb + 2 * int64_t(n);
   double2* part = reinterpret_cast<double2*>(st + kStateLen);
-  CgParams P{n, indptr, indices, nullptr, rtol, atol, dtol};
+  CgParams P{n, indptr, indices, nullptr, rtol, atol, dtol, trust, radius};
   const int nbv = n > 0 ? int(std::min<int64_t>(kVecBlocks, (int64_t(n) + kBlock - 1) / kBlock)) : 1;
   hipLaunchKernelGGL(cg_init_kernel, dim3(nbv), dim3(kBlock), 0, s, P, b, x, r, scratch_z, pa, part);
   FNE_HIP_CHECK(hipGraphLaunch(pc, s));
-  hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, z, r, nullptr, part);
+  hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, trust, z, r, nullptr, part);
   hipLaunchKernelGGL(cg_init_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
   FNE_LAUNCH_CHECK();
   double hst[kStateLen];
@@ -424,13 +514,13 @@ extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* in
       double* p_old = (it & 1) ? pb : pa;
       double* p_new = (it & 1) ? pa : pb;
       if (precision == FNE_FP32)
-        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, st, part, nbs, s);
+        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
       else
-        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, st, part, nbs, s);
-      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, part, nbs, st);
+        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+      hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, P, part, nbs, st);
       hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nbv), dim3(kBlock), 0, s, n, p_new, w, x, r, st);
       FNE_HIP_CHECK(hipGraphLaunch(pc, s));
-      hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, z, r, st, part);
+      hipLaunchKernelGGL(cg_dots_kernel, dim3(nbv), dim3(kBlock), 0, s, n, trust, z, r, st, part);
       hipLaunchKernelGGL(cg_beta_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
     }
     FNE_LAUNCH_CHECK();
@@ -442,4 +532,43 @@ extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* in
   if (reason_out) *reason_out = reason;
   if (rnorm_out) *rnorm_out = hst[kRnorm];
   return FNE_OK;
+}
+
+}  // namespace
+
+extern "C" int fne_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+                       const double* inv_diag, const double* b, double* x, double rtol, double atol, double dtol,
+                       int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+                       int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  return pcg_run("fne_pcg", 0, INFINITY, n, indptr, indices, vals, precision, inv_diag, b, x, rtol, atol, dtol, max_it,
+                 check_every, work, work_bytes, its_out, reason_out, rnorm_out, stream);
+}
+
+extern "C" int fne_pstcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
+                         int32_t precision, const double* inv_diag, const double* b, double* x, double radius,
+                         double rtol, double atol, double dtol, int32_t max_it, int32_t check_every, void* work,
+                         int64_t work_bytes, int32_t* its_out, int32_t* reason_out, double* rnorm_out,
+                         fne_stream_t stream) {
+  return pcg_run("fne_pstcg", 1, radius, n, indptr, indices, vals, precision, inv_diag, b, x, rtol, atol, dtol,
+                 max_it, check_every, work, work_bytes, its_out, reason_out, rnorm_out, stream);
+}
+
+extern "C" int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
+                             int32_t precision, void* pc_graph_exec, double* pc_in, const double* pc_out,
+                             const double* b, double* x, double rtol, double atol, double dtol, int32_t max_it,
+                             int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+                             int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  return pcg_graph_run("fne_pcg_graph", 0, INFINITY, n, indptr, indices, vals, precision, pc_graph_exec, pc_in,
+                       pc_out, b, x, rtol, atol, dtol, max_it, check_every, work, work_bytes, its_out, reason_out,
+                       rnorm_out, stream);
+}
+
+extern "C" int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
+                               int32_t precision, void* pc_graph_exec, double* pc_in, const double* pc_out,
+                               const double* b, double* x, double radius, double rtol, double atol, double dtol,
+                               int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+                               int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  return pcg_graph_run("fne_pstcg_graph", 1, radius, n, indptr, indices, vals, precision, pc_graph_exec, pc_in,
+                       pc_out, b, x, rtol, atol, dtol, max_it, check_every, work, work_bytes, its_out, reason_out,
+                       rnorm_out, stream);
 }

@@ -436,6 +436,49 @@ class KSP:
     # direction type), convergence on the unpreconditioned residual norm ||r|| (PETSc STCG's
     # default norm, the serial restatement's test), boundary / negative-curvature stops ----
     def _solve_stcg(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        # on the device (fne_pstcg / fne_pstcg_graph): the CG kernels in trust mode
+        import ctypes as C
+
+        amg = self._amg
+        if amg is not None and getattr(amg, "_graph", None) is None:
+            self._solve_stcg_host(b, x)
+            return
+        A = self.A
+        delta = float("inf") if self.radius is None else float(self.radius)
+        lib = native.lib()
+        nbytes = int(lib.fne_pcg_workspace_bytes(A.n, A.nnz))
+        if nbytes < 0:
+            native.check(-1, "fne_pcg_workspace_bytes")
+        if getattr(self, "_cg_work", None) is None or self._cg_work.numel() < nbytes:
+            self._cg_work = torch.empty(nbytes, dtype=torch.uint8, device=b.device)
+        its, reason, rnorm = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        if amg is not None:
+            amg.check_live()
+            if amg._A0 is not A:
+                raise RuntimeError("the KSP operator changed since the multigrid set-up; call setUp()")
+            native.check(
+                lib.fne_pstcg_graph(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data),
+                                    A.fne_precision, C.c_void_p(amg._graph.raw_cuda_graph_exec()),
+                                    native.ptr(amg._g_in), native.ptr(amg._g_out), native.ptr(b), native.ptr(x), delta,
+                                    self.rtol, self.atol, self.dtol, self.max_it, int(self.check_every_graph_pc),
+                                    native.ptr(self._cg_work), nbytes, C.byref(its), C.byref(reason), C.byref(rnorm),
+                                    native.stream_ptr()),
+                "fne_pstcg_graph",
+            )
+        else:
+            native.check(
+                lib.fne_pstcg(A.n, native.ptr(A.indptr), native.ptr(A.indices), native.ptr(A.data), A.fne_precision,
+                              native.ptr(self._inv_diag), native.ptr(b), native.ptr(x), delta, self.rtol, self.atol,
+                              self.dtol, self.max_it, int(self.check_every), native.ptr(self._cg_work), nbytes,
+                              C.byref(its), C.byref(reason), C.byref(rnorm), native.stream_ptr()),
+                "fne_pstcg",
+            )
+        self.its = int(its.value)
+        self.reason = int(reason.value)
+        self.rnorm = float(rnorm.value)
+
+    # the same Steihaug-Toint CG driven from the host (kept for checking the device solver)
+    def _solve_stcg_host(self, b: torch.Tensor, x: torch.Tensor) -> None:
         A = self.A
         delta = float("inf") if self.radius is None else float(self.radius)
         x.zero_()
@@ -492,6 +535,9 @@ class KSP:
             if self.reason:
                 return
             rz_new = self._dot(r, z)
+            if rz_new < 0.0:
+                self.reason = KSP_DIVERGED_INDEFINITE_PC
+                return
             p.mul_(rz_new / rz).add_(z)
             rz = rz_new
         self.reason = KSP_DIVERGED_ITS

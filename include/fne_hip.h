@@ -269,6 +269,27 @@ int fne_pcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, cons
                   double atol, double dtol, int32_t max_it, int32_t check_every, void* work, int64_t work_bytes,
                   int32_t* its_out, int32_t* reason_out, double* rnorm_out, fne_stream_t stream);
 
+/*
+ * Steihaug-Toint CG (PETSc KSPSTCG, ksp_type "stcg"), the trust-region subproblem solve:
+ * approximately minimise -b.x + x.A x / 2 over ||x||_2 <= radius (radius >= 0, INFINITY =
+ * no bound).  Replaces ksp.solve after KSPCGSetRadius (core/petsc/trust_ksp.py:55-72,
+ * core/petsc/scalar_problem_driver.py:143-170; the reference's serial restatement is
+ * core/serial/sparse_solvers.py:20-93).  Same arguments, workspace and outputs as fne_pcg /
+ * fne_pcg_graph plus the radius; differences: convergence on the unpreconditioned residual
+ * norm ||r|| (also the reported norm); curvature p.Ap <= 0 moves x along p to the sphere
+ * and returns reason 5 (with radius = INFINITY x stays at the last iterate); a step that
+ * would leave the ball moves x to the sphere and returns reason 6.
+ */
+int fne_pstcg(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+              const double* inv_diag, const double* b, double* x, double radius, double rtol, double atol,
+              double dtol, int32_t max_it, int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out,
+              int32_t* reason_out, double* rnorm_out, fne_stream_t stream);
+int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+                    void* pc_graph_exec, double* pc_in, const double* pc_out, const double* b, double* x,
+                    double radius, double rtol, double atol, double dtol, int32_t max_it, int32_t check_every,
+                    void* work, int64_t work_bytes, int32_t* its_out, int32_t* reason_out, double* rnorm_out,
+                    fne_stream_t stream);
+
 /* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
 int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
 

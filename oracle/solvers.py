@@ -177,6 +177,8 @@ def stcg(A, b, radius=math.inf, rtol=1e-3, max_it=10000, atol=1e-50, dtol=1e5, i
         if dp >= dtol * r0:
             return x, i + 1, -4
         rz_new = float(r @ z)
+        if rz_new < 0.0:
+            return x, i + 1, -8
         p = z + (rz_new / rz) * p
         rz = rz_new
     return x, max_it, -3

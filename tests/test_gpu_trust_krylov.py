@@ -3,10 +3,11 @@
 * GMRES(restart) over the device CSR operator: same iteration count and reason as the
   oracle's KSPGMRES restatement, solution to 1e-10 relative, on an assembled p-Laplace
   Hessian and on a nonsymmetric sparse matrix.
-* STCG (the trust-region subproblem solver, ksp_type "stcg"): same iteration count and
-  reason as the oracle for no radius, a radius below the first step, a radius the path
-  crosses, and negative curvature (the Ginzburg-Landau Hessian at u = 0 is indefinite);
-  solutions to 1e-10 relative; boundary stops end on the sphere.
+* STCG (the trust-region subproblem solver, ksp_type "stcg", on the device: fne_pstcg, and
+  with the multigrid V-cycle graph: fne_pstcg_graph): same iteration count and reason as
+  the oracle and as the host-driven restatement for no radius, a radius below the first
+  step, a radius the path crosses, and negative curvature (the Ginzburg-Landau Hessian at
+  u = 0 is indefinite); solutions to 1e-10 relative; boundary stops end on the sphere.
 * trust-region Newton on the reference p-Laplace L5 mesh, with STCG subproblems and with the
   reference's 1-D / 2-D model subproblems: both converge to the energy of line-search Newton
   and to the reference's published value (p-Laplace is strictly convex: one minimiser).
@@ -22,6 +23,7 @@ import torch
 from fenics_nonlinear_energies_amd.drivers import solve_newton
 from fenics_nonlinear_energies_amd.linalg import KSP, DeviceCSR
 from fenics_nonlinear_energies_amd.problems.ginzburg_landau import GLReorderedElementAssembler, build_gl_problem
+from fenics_nonlinear_energies_amd.problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
 from fenics_nonlinear_energies_amd.problems.plaplace import (
     PLaplaceReorderedElementAssembler,
     build_plaplace_problem,
@@ -130,6 +132,63 @@ def test_device_stcg_negative_curvature_gl(require_gpu):
         assert (its, reason) == (its_o, reason_o)
         assert abs(np.linalg.norm(x) - delta) <= 1e-10 * delta
         assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+    asm.cleanup()
+
+
+def test_device_stcg_matches_host_restatement(require_gpu):
+    """fne_pstcg (the device iteration) against the host-driven Steihaug-Toint restatement:
+    same iteration count and reason, solution to 1e-10, for the radii above and no radius."""
+    asm = _plaplace_hessian()
+    ksp = KSP(ksp_type="stcg", pc_type="jacobi", rtol=1e-6, max_it=10000)
+    ksp.setOperators(asm.A)
+    ksp.setUp()
+    b = _vec(np.random.default_rng(4).standard_normal(asm.layout.n_free))
+    full = None
+    for frac in (None, 1e-4, 0.3, 0.9):
+        ksp.radius = None if frac is None else frac * full
+        xd = _vec(np.zeros(asm.layout.n_free))
+        ksp.solve(b, xd)
+        its_d, reason_d = ksp.getIterationNumber(), ksp.getConvergedReason()
+        xh = torch.empty_like(b.t)
+        ksp._solve_stcg_host(b.t, xh)
+        assert (its_d, reason_d) == (ksp.getIterationNumber(), ksp.getConvergedReason()), (frac, its_d, reason_d)
+        torch.testing.assert_close(xd.t, xh, rtol=1e-10, atol=1e-10 * float(xh.abs().max()))
+        if frac is None:
+            full = float(xh.norm())
+        else:
+            assert reason_d == 6 and abs(float(xd.t.norm()) - ksp.radius) <= 1e-10 * ksp.radius
+    asm.cleanup()
+
+
+def test_device_stcg_with_graph_preconditioner_matches_host(require_gpu):
+    """fne_pstcg_graph (V-cycle graph launched from the device iteration) against the host
+    restatement with the same multigrid: same iterations and reason, solution to 1e-10."""
+    p, adj, u0 = build_beam_problem(40, 4, 4)
+    asm = HEReorderedElementAssembler(p, None, adj, ksp_type="stcg", pc_type="gamg")
+    x = asm.create_vec((u0 + 1e-5 * np.random.default_rng(4).standard_normal(u0.size))[asm.layout.perm])
+    asm.assemble_hessian(x)
+    ksp = asm.ksp
+    ksp.setOperators(asm.A)
+    ksp.setUp()
+    assert ksp._amg._graph is not None
+    b = asm.create_vec(np.random.default_rng(5).standard_normal(asm.layout.n_free))
+    ksp.setTolerances(rtol=1e-8, max_it=500)
+    ksp.radius = None
+    xf = b.duplicate()
+    ksp.solve(b, xf)
+    assert ksp.getConvergedReason() == 2
+    full = xf.norm()
+    for radius, rtol in ((None, 1e-1), (None, 1e-8), (0.5 * full, 1e-8), (1e-3 * full, 1e-8)):
+        ksp.setTolerances(rtol=rtol, max_it=500)
+        ksp.radius = radius
+        xd = b.duplicate()
+        ksp.solve(b, xd)
+        its_d, reason_d = ksp.getIterationNumber(), ksp.getConvergedReason()
+        xh = torch.empty_like(b.t)
+        ksp._solve_stcg_host(b.t, xh)
+        assert (its_d, reason_d) == (ksp.getIterationNumber(), ksp.getConvergedReason()), (radius, its_d, reason_d)
+        assert reason_d == (2 if radius is None else 6)
+        torch.testing.assert_close(xd.t, xh, rtol=1e-10, atol=1e-10 * float(xh.abs().max()))
     asm.cleanup()
 
 
