@@ -52,7 +52,18 @@ def he_predictor(p, asm, net, mode, total_angle):
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
                        ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=200, lr=2e-3,
-                       seed=0, guard=True, with_reference=True, ksp_forcing=None, newton_kwargs=None, log=print):
+                       seed=0, guard=True, with_reference=True, with_trust_reference=False, ksp_forcing=None,
+                       newton_kwargs=None, log=print):
+    """Load-step time to tolerance on ``eval_mesh``: reference profiles (FP64) against previous /
+    extrapolated / learned initial guesses (FP32 Hessian), after training the nodal MLP on runs
+    of ``train_mesh``.
+
+    ``with_trust_reference`` adds ``reference_trust_fp64``: the solver settings of the
+    reference's best Neo-Hookean campaign (experiments/runners/run_he_final_suite_best.py:
+    123-173): trust-region Newton with Steihaug-Toint CG subproblems (radius 0.5 at start) and
+    a line search on [-0.5, 2] (tol 1e-1) inside the radius, multigrid rebuilt at every Newton
+    iteration, KSP rtol 1e-1 capped at 30 iterations; stopping tolerances as for the others.
+    """
     from .problems.hyperelasticity import solve_he_load_steps
 
     dims = [10] + [hidden] * layers + [3]
@@ -84,16 +95,21 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1],
            "train_time_s": t_train, "data_gen_time_s": t_gen, "variants": {}}
     E, nu = eval_material
-    # (name, initial guess, Hessian precision, keep the PC, rebuild above this many its, KSP cap)
-    variants = [("previous_fp64", "previous", "fp64", pc_reuse, pc_refresh_its, ksp_max_it),
-                ("extrapolate_fp32", "extrapolate", "fp32", pc_reuse, pc_refresh_its, ksp_max_it),
-                ("learned_fp32", "learned", "fp32", pc_reuse, pc_refresh_its, ksp_max_it)]
+    # (name, initial guess, Hessian precision, keep the PC, rebuild above this many its, KSP cap,
+    #  KSP type, extra Newton options)
+    variants = [("previous_fp64", "previous", "fp64", pc_reuse, pc_refresh_its, ksp_max_it, ksp_type, {}),
+                ("extrapolate_fp32", "extrapolate", "fp32", pc_reuse, pc_refresh_its, ksp_max_it, ksp_type, {}),
+                ("learned_fp32", "learned", "fp32", pc_reuse, pc_refresh_its, ksp_max_it, ksp_type, {})]
+    if with_trust_reference:
+        trust = dict(use_trust_region=True, trust_radius_init=0.5, trust_subproblem_line_search=True,
+                     linesearch_tol=1e-1)
+        variants.insert(0, ("reference_trust_fp64", "previous", "fp64", False, None, 30, "stcg", trust))
     if with_reference:
         # the reference's "reference" profile on the GPU (hyperelasticity/jax_petsc/solver.py:35-46):
         # previous state, FP64, KSP capped at 30 iterations, PC kept and rebuilt at the cap
-        variants.insert(0, ("reference_fp64", "previous", "fp64", True, None, 30))
-    for name, mode, prec, reuse, refresh, max_it in variants:
-        p, asm = _he_assembler(*eval_mesh, E, nu, prec, max_it, ksp_type, pc_type)
+        variants.insert(0, ("reference_fp64", "previous", "fp64", True, None, 30, ksp_type, {}))
+    for name, mode, prec, reuse, refresh, max_it, v_ksp, v_newton in variants:
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, max_it, v_ksp, pc_type)
         pred = he_predictor(p, asm, net, mode, total_angle)
         pred.feat_mean, pred.feat_std, pred.targ_std = stats
         if guard:
@@ -103,7 +119,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         recs, x = solve_he_load_steps(
             asm, total_steps, total_angle, predictor=pred, ksp_max_it=max_it, ksp_rtol=ksp_rtol,
             pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh,
-            newton_kwargs=dict(newton_kwargs or {}, **({} if name == "reference_fp64" else {"ksp_forcing": ksp_forcing})),
+            newton_kwargs=dict(newton_kwargs or {}, **v_newton,
+                               **({} if name.startswith("reference") else {"ksp_forcing": ksp_forcing})),
             on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
