@@ -5,7 +5,8 @@
 * ``KSP`` reproduces the PETSc KSP semantics the reference relies on -- zero initial guess,
   left preconditioning, convergence on the preconditioned residual norm
   ``||B r_k|| <= max(rtol*||B r_0||, abstol)``, divergence at ``dtol``, iteration cap -- for
-  ``cg`` (KSPCG) and ``gmres`` (KSPGMRES, restart 30, modified Gram-Schmidt), and ``stcg``
+  ``cg`` (KSPCG) and ``gmres`` (KSPGMRES, restart 30, classical Gram-Schmidt without
+  refinement as PETSc's default orthogonalization), and ``stcg``
   (Steihaug-Toint CG with a radius, as set by KSPCGSetRadius in core/petsc/trust_ksp.py),
   which tests the unpreconditioned residual norm ``||r_k||`` instead, as PETSc's KSPSTCG does
   by default.
@@ -350,7 +351,7 @@ class KSP:
                 return
         self.reason = KSP_DIVERGED_ITS
 
-    # KSPGMRES(restart), left preconditioning, modified Gram-Schmidt ----------------------
+    # KSPGMRES(restart), left preconditioning, classical Gram-Schmidt ---------------------
     def _solve_gmres(self, b: torch.Tensor, x: torch.Tensor) -> None:
         A = self.A
         m = max(1, self.restart)
@@ -389,11 +390,13 @@ class KSP:
             for k in range(m):
                 A.mult_tensor(V[k], tmp)
                 self._pc(tmp, w)
-                for j in range(k + 1):
-                    h = self._dot(w, V[j])
-                    H[j, k] = h
-                    w.add_(V[j], alpha=-h)
-                hn = math.sqrt(max(self._dot(w, w), 0.0))
+                # classical Gram-Schmidt, PETSc's default (all projections of the same w, no
+                # refinement): one multi-dot, one multi-axpy and one read-back per iteration
+                hk = torch.mv(V[: k + 1], w)
+                w.sub_(torch.mv(V[: k + 1].T, hk))
+                hv = torch.cat((hk, torch.dot(w, w).view(1))).cpu().numpy()
+                H[: k + 1, k] = hv[: k + 1]
+                hn = math.sqrt(max(float(hv[k + 1]), 0.0))
                 H[k + 1, k] = hn
                 if hn > 0.0:
                     torch.mul(w, 1.0 / hn, out=V[k + 1])

@@ -3,7 +3,9 @@
 * ``cg``: PETSc KSPCG semantics (zero initial guess, left Jacobi preconditioning with zero
   diagonal entries replaced by 1, convergence on the preconditioned residual norm
   ||B r_k|| <= rtol ||B r_0||, indefinite-curvature stop, iteration cap).
-* ``gmres``: PETSc KSPGMRES(restart) with left preconditioning, modified Gram-Schmidt.
+* ``gmres``: PETSc KSPGMRES(restart) with left preconditioning and its default
+  orthogonalization, classical Gram-Schmidt without refinement
+  (KSPGMRESClassicalGramSchmidtOrthogonalization, KSP_GMRES_CGS_REFINE_NEVER).
 * ``stcg``: PETSc KSPSTCG, the trust-region (Steihaug-Toint) CG of the trust-region Newton.
 * ``newton_golden``: the line-search branch of core/petsc/minimizers.py:181-1038
   (golden-section search on [a, b] after non-finite interval repair, convergence on
@@ -86,9 +88,8 @@ def gmres(A, b, rtol=1e-3, max_it=10000, restart=30, atol=1e-50, inv_diag=None):
         done = 0
         for k in range(m):
             w = inv_diag * (A @ V[k])
-            for j in range(k + 1):
-                H[j, k] = float(w @ V[j])
-                w -= H[j, k] * V[j]
+            H[: k + 1, k] = V[: k + 1] @ w  # classical Gram-Schmidt
+            w = w - H[: k + 1, k] @ V[: k + 1]
             hn = math.sqrt(float(w @ w))
             H[k + 1, k] = hn
             if hn > 0.0:
