@@ -238,3 +238,22 @@ def test_trust_region_load_steps_he_level1_published_energy(require_gpu):
     assert all("onverged" in r["message"] for r in recs), [r["message"] for r in recs]
     assert abs(recs[-1]["energy"] - pub) <= 2e-4 * abs(pub)
     asm.cleanup()
+
+
+def test_stcg_rejects_a_negative_radius(require_gpu):
+    """KSPCGSetRadius requires radius >= 0: a negative or NaN radius is an argument error
+    (FNE_EINVAL, raised as RuntimeError by the KSP), checked before anything is launched."""
+    asm = _plaplace_hessian(16)
+    ksp = KSP(ksp_type="stcg", pc_type="jacobi", rtol=1e-6)
+    ksp.setOperators(asm.A)
+    ksp.setUp()
+    b = _vec(np.ones(asm.layout.n_free))
+    for bad in (-1.0, float("nan")):
+        ksp.setCGRadius(bad)
+        with pytest.raises(RuntimeError, match="fne_pstcg"):
+            ksp.solve(b, _vec(np.zeros(asm.layout.n_free)))
+    ksp.setCGRadius(0.0)  # zero radius: the step stays at 0 on the boundary
+    x = _vec(np.ones(asm.layout.n_free))
+    ksp.solve(b, x)
+    assert ksp.getConvergedReason() == 6 and float(x.t.abs().max()) == 0.0
+    asm.cleanup()
