@@ -1,0 +1,172 @@
+"""The trust-region branch of the Newton driver (minimizers.newton) on the CPU.
+
+The driver only needs vectors with the DeviceVec methods, so these tests run it on a small
+numpy-backed vector with the oracle's Steihaug CG as the subproblem solver, and check what the
+rho rule of core/petsc/minimizers.py:181-1038 fixes:
+
+* on a quadratic the model is exact (rho = 1): every step is accepted, a step that reaches the
+  radius expands it by trust_expand, steps never leave the radius, the energy falls
+  monotonically and Newton ends at the minimiser;
+* when the model overshoots (a quartic energy solved with its quadratic part only), the first
+  steps give rho < eta_shrink: they are rejected and the radius is multiplied by trust_shrink
+  until a step is accepted, or, after 1 + trust_max_reject rejections, the driver stops and
+  keeps the iterate;
+* the 1-D / 2-D model branch (a non-trust KSP) reaches the same minimiser.
+"""
+
+import math
+
+import numpy as np
+import scipy.sparse as sp
+
+from fenics_nonlinear_energies_amd.minimizers import newton
+from oracle import solvers
+
+
+class NumpyVec:
+    """The DeviceVec interface over a numpy array (test helper)."""
+
+    def __init__(self, a):
+        self.a = np.array(a, dtype=np.float64)
+
+    def duplicate(self):
+        return NumpyVec(np.empty_like(self.a))
+
+    def destroy(self):
+        self.a = None
+
+    @property
+    def array(self):
+        return self.a.copy()
+
+    def copy(self, other=None):
+        if other is None:
+            return NumpyVec(self.a)
+        other.a[:] = self.a
+        return other
+
+    def set(self, v):
+        self.a[:] = v
+
+    def scale(self, s):
+        self.a *= s
+
+    def axpy(self, s, x):
+        self.a += s * x.a
+
+    def aypx(self, s, x):
+        self.a[:] = s * self.a + x.a
+
+    def waxpy(self, s, x, y):
+        self.a[:] = s * x.a + y.a
+
+    def dot(self, o):
+        return float(self.a @ o.a)
+
+    def norm(self, norm_type=None):
+        return float(np.linalg.norm(self.a))
+
+
+def _problem(n=30, seed=0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = sp.csr_matrix((q * np.geomspace(1.0, 20.0, n)) @ q.T)
+    b = rng.standard_normal(n)
+    return A, b
+
+
+def _run(A, b, energy, grad, use_stcg=True, **kw):
+    one = np.ones(b.size)
+    calls = {"radii": []}
+
+    def trust_solve(x, rhs, sol, radius):
+        calls["radii"].append(radius)
+        s, its, _ = solvers.stcg(A, rhs.a, radius=radius, rtol=1e-12, inv_diag=one)
+        sol.a[:] = s
+        return its
+
+    def hess_solve(x, rhs, sol):
+        s, its, _ = solvers.cg(A, rhs.a, rtol=1e-12, inv_diag=one)
+        sol.a[:] = s
+        return its
+
+    def matvec(x, vin, vout):
+        vout.a[:] = A @ vin.a
+
+    x = NumpyVec(np.zeros(b.size))
+    res = newton(energy_fn=lambda v: energy(v.a), gradient_fn=lambda v, g: g.a.__setitem__(slice(None), grad(v.a)),
+                 hessian_solve_fn=hess_solve, x=x, hessian_matvec_fn=matvec, save_history=True, trust_region=True,
+                 trust_subproblem_solve_fn=trust_solve if use_stcg else None, **kw)
+    return res, x, calls
+
+
+def test_quadratic_model_is_exact_radius_expands_until_the_newton_step_fits():
+    A, b = _problem()
+    xs = np.linalg.solve(A.toarray(), b)
+    E = lambda u: 0.5 * u @ (A @ u) - b @ u
+    G = lambda u: A @ u - b
+    r0 = 0.05 * np.linalg.norm(xs)
+    res, x, calls = _run(A, b, E, G, tolf=1e-14, tolg=1e-9, tolx_rel=0.0, tolx_abs=0.0, maxit=60,
+                         trust_radius_init=r0, require_all_convergence=False)
+    hist = res["history"]
+    assert "converged" in res["message"].lower(), res["message"]
+    assert np.abs(x.a - xs).max() <= 1e-8 * np.abs(xs).max()
+    assert all(h["accepted_step"] and h["trust_rejects"] == 0 for h in hist)
+    assert all(abs(h["trust_ratio"] - 1.0) < 1e-6 for h in hist if h["pred_reduction"] > 1e-10)
+    energies = [h["energy"] for h in hist]
+    assert all(e1 <= e0 for e0, e1 in zip(energies, energies[1:]))
+    # every step stays in its radius; a boundary step multiplies the radius by 1.5
+    radii = calls["radii"]
+    for k, h in enumerate(hist):
+        assert h["step_norm"] <= radii[k] * (1 + 1e-12)
+        if k + 1 < len(radii):
+            expect = radii[k] * 1.5 if h["step_norm"] >= 0.9 * radii[k] else radii[k]
+            assert math.isclose(radii[k + 1], expect, rel_tol=1e-12)
+    assert math.isclose(hist[0]["step_norm"], r0, rel_tol=1e-12) and radii[1] == 1.5 * r0
+    assert len(hist) >= 2 + math.floor(math.log(np.linalg.norm(xs) / r0, 1.5) / 2)
+
+
+def _quartic(seed=1, c=50.0):
+    A, b = _problem(seed=seed)
+    E = lambda u: 0.5 * u @ (A @ u) - b @ u + c * np.sum(u**4)
+    G = lambda u: A @ u - b + 4.0 * c * u**3
+    return A, b, E, G
+
+
+def test_overshooting_model_rejects_and_shrinks():
+    """E = quadratic + 50 sum(u^4), modelled by its quadratic part: from radius 2 the first two
+    trial steps overshoot (rho < 0.05) and are rejected, halving the radius each time; the third
+    (radius 0.5) is accepted."""
+    A, b, E, G = _quartic()
+    res, x, calls = _run(A, b, E, G, tolf=1e-12, tolg=1e-6, tolx_rel=0.0, tolx_abs=0.0, maxit=200,
+                         trust_radius_init=2.0, require_all_convergence=False)
+    hist = res["history"]
+    radii = calls["radii"]
+    assert hist[0]["trust_rejects"] == 2 and hist[0]["accepted_step"]
+    assert radii[:3] == [2.0, 1.0, 0.5]
+    assert all(h["accepted_step"] for h in hist)
+    energies = [h["energy"] for h in hist]
+    assert all(e1 <= e0 for e0, e1 in zip(energies, energies[1:]))
+    assert np.linalg.norm(G(x.a)) < 1e-5
+
+
+def test_all_candidates_rejected_keeps_the_iterate():
+    """From radius 100 every one of the 1 + trust_max_reject = 7 trial steps is rejected: the
+    radius halves each time, the driver stops with the reference's message, x is unchanged."""
+    A, b, E, G = _quartic()
+    res, x, calls = _run(A, b, E, G, tolf=1e-12, tolg=1e-6, tolx_rel=0.0, tolx_abs=0.0, maxit=200,
+                         trust_radius_init=100.0, require_all_convergence=False)
+    assert res["message"] == "Trust-region rejected all candidate steps at Newton iteration 1"
+    assert calls["radii"] == [100.0 * 0.5**k for k in range(7)]
+    assert not x.a.any()
+
+
+def test_model_branch_without_trust_ksp_reaches_the_minimiser():
+    A, b = _problem(seed=2)
+    xs = np.linalg.solve(A.toarray(), b)
+    E = lambda u: 0.5 * u @ (A @ u) - b @ u
+    G = lambda u: A @ u - b
+    res, x, _ = _run(A, b, E, G, tolf=1e-14, tolg=1e-9, tolx_rel=0.0, tolx_abs=0.0, maxit=80,
+                     trust_radius_init=0.1, require_all_convergence=False, use_stcg=False)
+    assert "converged" in res["message"].lower(), res["message"]
+    assert np.abs(x.a - xs).max() <= 1e-7 * np.abs(xs).max()
