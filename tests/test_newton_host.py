@@ -1,8 +1,9 @@
-"""The trust-region branch of the Newton driver (minimizers.newton) on the CPU.
+"""The Newton driver's control flow (minimizers.newton) on the CPU.
 
 The driver only needs vectors with the DeviceVec methods, so these tests run it on a small
-numpy-backed vector with the oracle's Steihaug CG as the subproblem solver, and check what the
-rho rule of core/petsc/minimizers.py:181-1038 fixes:
+numpy-backed vector.  The golden-section branch equals the oracle's Newton iterate for
+iterate.  The trust-region branch, with the oracle's Steihaug CG as the subproblem solver,
+does what the rho rule of core/petsc/minimizers.py:181-1038 fixes:
 
 * on a quadratic the model is exact (rho = 1): every step is accepted, a step that reaches the
   radius expands it by trust_expand, steps never leave the radius, the energy falls
@@ -170,3 +171,36 @@ def test_model_branch_without_trust_ksp_reaches_the_minimiser():
                      trust_radius_init=0.1, require_all_convergence=False, use_stcg=False)
     assert "converged" in res["message"].lower(), res["message"]
     assert np.abs(x.a - xs).max() <= 1e-7 * np.abs(xs).max()
+
+
+def test_line_search_branch_equals_the_oracle_newton():
+    """The golden-section branch of the same driver against the oracle's Newton on a p-Laplace
+    problem (oracle energy, gradient and Hessian; Jacobi CG): the same iterations, iterates and
+    energy, since both evaluate the same points in the same order."""
+    from fenics_nonlinear_energies_amd.problems.plaplace import build_plaplace_problem
+    from oracle import assembly
+
+    params, _, _ = build_plaplace_problem(12)
+    n = params["freedofs"].size
+    u0 = np.random.default_rng(3).random(n)
+    E = lambda u: assembly.assemble("plaplace", params, u, want=("e",))["e"]
+    G = lambda u: assembly.assemble("plaplace", params, u, want=("g",))["g"]
+
+    def S(u, rhs):
+        H = assembly.assemble("plaplace", params, u, want=("h",))["h"]
+        y, its, _ = solvers.cg(H, rhs, rtol=1e-3)
+        return y, its
+
+    def hess_solve(x, rhs, sol):
+        y, its = S(x.a, rhs.a)
+        sol.a[:] = y
+        return its
+
+    ref = solvers.newton_golden(E, G, S, u0, tolf=1e-5, tolg=1e-3)
+    x = NumpyVec(u0)
+    res = newton(energy_fn=lambda v: E(v.a), gradient_fn=lambda v, g: g.a.__setitem__(slice(None), G(v.a)),
+                 hessian_solve_fn=hess_solve, x=x, tolf=1e-5, tolg=1e-3, require_all_convergence=True)
+    assert "converged" in res["message"].lower(), res["message"]
+    assert res["nit"] == ref["nit"]
+    assert abs(res["fun"] - ref["fun"]) <= 1e-13 * abs(ref["fun"])
+    assert np.abs(x.a - ref["x"]).max() <= 1e-11 * np.abs(ref["x"]).max()
