@@ -1,0 +1,150 @@
+"""Host logic of the Newton driver's linear solves: the preconditioner set-up policy and the
+Eisenstat-Walker forcing term.
+
+Set-up policy: by default before every solve; with pc_setup_on_ksp_cap only on the first
+solve and after a solve that hit ksp_max_it (core/petsc/scalar_problem_driver.py:150-202),
+here also after one above pc_refresh_its, and keep_pc skips the first set-up when a
+preconditioner exists.
+
+Forcing term: eta_0 = ksp_rtol; eta_k = gamma (|g_k| / |g_{k-1}|)^alpha with gamma = 0.9, alpha = 2,
+raised to gamma eta_{k-1}^alpha when that exceeds 0.1, capped by ksp_rtol and floored at
+1e-8 (Eisenstat & Walker 1996, choice 2); the fixed policy always returns ksp_rtol.
+"""
+
+import pytest
+
+from fenics_nonlinear_energies_amd.drivers import LinearSolveContext
+
+
+class _Vec:
+    def __init__(self, n):
+        self.n = n
+
+    def norm(self):
+        return self.n
+
+
+class _Asm:
+    class ksp:  # noqa: N801 - attribute bag standing in for the assembler's KSP
+        _amg = None
+        _inv_diag = None
+
+
+def _ctx(forcing, rtol=0.1):
+    return LinearSolveContext(_Asm(), ksp_rtol=rtol, ksp_max_it=100, ksp_forcing=forcing)
+
+
+def test_fixed_policy_returns_ksp_rtol():
+    c = _ctx(None)
+    assert [c._rtol_for(_Vec(g)) for g in (10.0, 1.0, 1e-3)] == [0.1, 0.1, 0.1]
+
+
+def test_eisenstat_walker_sequence():
+    c = _ctx("ew")
+    assert c._rtol_for(_Vec(1.0)) == 0.1  # first solve: ksp_rtol
+    # fast decrease: 0.9 * (0.01)^2 = 9e-5, safeguard 0.9 * 0.1^2 = 9e-3 <= 0.1 -> not applied
+    assert c._rtol_for(_Vec(0.01)) == pytest.approx(9e-5)
+    # slow decrease: 0.9 * 0.9^2 = 0.729 -> capped by ksp_rtol
+    assert c._rtol_for(_Vec(0.009)) == pytest.approx(0.1)
+    # after eta = 0.1 the safeguard 0.9 * 0.1^2 = 9e-3 is below 0.1, so the plain term holds
+    assert c._rtol_for(_Vec(0.009 * 1e-3)) == pytest.approx(max(0.9 * 1e-6, 1e-8))
+    c.reset()
+    assert c._rtol_for(_Vec(5.0)) == 0.1  # reset starts over
+
+
+def test_eisenstat_walker_safeguard_and_floor():
+    c = _ctx("ew", rtol=0.9)
+    c._rtol_for(_Vec(1.0))  # eta_0 = 0.9
+    # plain term 0.9 * (1e-6)^2 is tiny, safeguard 0.9 * 0.9^2 = 0.729 > 0.1 keeps it up
+    assert c._rtol_for(_Vec(1e-6)) == pytest.approx(0.729)
+    c2 = _ctx("ew", rtol=0.05)
+    c2._rtol_for(_Vec(1.0))
+    assert c2._rtol_for(_Vec(1e-9)) == pytest.approx(1e-8)  # floor
+
+
+def test_unknown_forcing_rejected():
+    with pytest.raises(ValueError):
+        _ctx("bogus")
+
+
+class _ScriptedKsp:
+    """Records set-ups and radii; each solve reports the next scripted iteration count."""
+
+    def __init__(self, its, has_pc=False):
+        self.script = list(its)
+        self.setups = []
+        self.radii = []
+        self.k = 0
+        self._amg = object() if has_pc else None
+        self._inv_diag = None
+        self._its = 0
+
+    def setCGRadius(self, r):
+        self.radii.append(r)
+
+    def setOperators(self, A):
+        pass
+
+    def setTolerances(self, rtol=None, max_it=None):
+        self.max_it = max_it
+
+    def setUp(self):
+        self.setups.append(self.k)
+
+    def solve(self, rhs, sol):
+        self._its = self.script[self.k]
+        self.k += 1
+
+    def getIterationNumber(self):
+        return self._its
+
+    def getConvergedReason(self):
+        return 2
+
+
+class _ScriptedAsm:
+    def __init__(self, ksp):
+        self.ksp = ksp
+        self.A = None
+
+    def assemble_hessian(self, vec):
+        pass
+
+
+def _solves(its, **kw):
+    ksp = _ScriptedKsp(its, has_pc=kw.pop("has_pc", False))
+    c = LinearSolveContext(_ScriptedAsm(ksp), ksp_rtol=0.1, ksp_max_it=30, **kw)
+    for _ in its:
+        c.hessian_solve_fn(None, _Vec(1.0), None)
+    return ksp.setups, c.iters
+
+
+def test_pc_set_up_every_solve_by_default():
+    """pc_setup_on_ksp_cap off (core/petsc/scalar_problem_driver.py:186-192): set up before
+    every solve."""
+    setups, iters = _solves([5, 30, 7, 12])
+    assert setups == [0, 1, 2, 3] and iters == [5, 30, 7, 12]
+
+
+def test_pc_rebuilt_only_after_a_solve_hits_the_cap():
+    """pc_setup_on_ksp_cap (scalar_problem_driver.py:186-202): set up on the first solve, then
+    only after a solve reached ksp_max_it (30)."""
+    setups, _ = _solves([5, 30, 7, 12, 31, 4], pc_setup_on_ksp_cap=True)
+    assert setups == [0, 2, 5]
+
+
+def test_pc_refresh_above_a_threshold_and_kept_across_calls():
+    """Additionally rebuilt after a solve above pc_refresh_its; keep_pc carries an existing
+    preconditioner over, so the first solve sets nothing up."""
+    setups, _ = _solves([5, 21, 7, 20, 22, 3], pc_setup_on_ksp_cap=True, pc_refresh_its=20)
+    assert setups == [0, 2, 5]
+    setups, _ = _solves([5, 21, 7], pc_setup_on_ksp_cap=True, pc_refresh_its=20, keep_pc=True, has_pc=True)
+    assert setups == [2]
+
+
+def test_trust_radius_reaches_the_ksp():
+    ksp = _ScriptedKsp([3, 4])
+    c = LinearSolveContext(_ScriptedAsm(ksp), ksp_rtol=0.1, ksp_max_it=30)
+    c.trust_subproblem_solve_fn(None, _Vec(1.0), None, 0.5)
+    c.hessian_solve_fn(None, _Vec(1.0), None)
+    assert ksp.radii == [0.5]
