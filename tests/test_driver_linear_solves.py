@@ -1,5 +1,6 @@
-"""Host logic of the Newton driver's linear solves: the preconditioner set-up policy and the
-Eisenstat-Walker forcing term.
+"""Host logic around the Newton driver: the preconditioner set-up policy and the
+Eisenstat-Walker forcing term of the linear solves, the retry schedule and the line-search
+interval repair.
 
 Set-up policy: by default before every solve; with pc_setup_on_ksp_cap only on the first
 solve and after a solve that hit ksp_max_it (core/petsc/scalar_problem_driver.py:150-202),
@@ -148,3 +149,47 @@ def test_trust_radius_reaches_the_ksp():
     c.trust_subproblem_solve_fn(None, _Vec(1.0), None, 0.5)
     c.hessian_solve_fn(None, _Vec(1.0), None)
     assert ksp.radii == [0.5]
+
+
+def test_retry_schedule():
+    """Primary attempt, plus with retry a repair attempt: rtol x 0.1 (floored at 1e-12), the
+    line-search interval's upper end capped at 1, the KSP cap doubled (core/benchmark/
+    repair.py:29-80); nothing to tighten -> no repair attempt."""
+    from fenics_nonlinear_energies_amd.repair import build_retry_attempts
+
+    base = dict(linesearch_interval=(-0.5, 2.0), ksp_rtol=1e-1, ksp_max_it=30)
+    assert build_retry_attempts(**base) == [("primary", (-0.5, 2.0), 1e-1, 30)]
+    att = build_retry_attempts(retry_on_failure=True, **base)
+    assert att[1][0] == "repair" and att[1][1] == (-0.5, 1.0) and att[1][3] == 60
+    assert att[1][2] == pytest.approx(1e-2)
+    assert build_retry_attempts(retry_on_nonfinite=False, retry_on_maxit=False, retry_on_failure=True, **base) == [
+        ("primary", (-0.5, 2.0), 1e-1, 30)]
+    tiny = build_retry_attempts(retry_on_failure=True, linesearch_interval=(-0.5, 0.5), ksp_rtol=1e-12, ksp_max_it=1)
+    assert tiny[1] == ("repair", (-0.5, 0.5), 1e-12, 2)
+
+
+def test_which_results_trigger_a_repair():
+    from fenics_nonlinear_energies_amd.repair import needs_solver_repair
+
+    assert needs_solver_repair({"fun": float("nan"), "message": "Converged"})
+    assert needs_solver_repair({"fun": 1.0, "message": "Non-finite energy after update at Newton iteration 3"})
+    assert needs_solver_repair({"fun": 1.0, "message": "Maximum number of iterations reached"})
+    assert not needs_solver_repair({"fun": 1.0, "message": "Maximum number of iterations reached"},
+                                   retry_on_maxit=False)
+    assert not needs_solver_repair({"fun": 1.0, "message": "Converged (energy, step, gradient)"})
+
+
+def test_line_search_interval_repair_bisects_to_the_finite_region():
+    """f is finite for alpha < 0.7 and for alpha > -0.3 only: both ends of [-0.5, 2] are pulled
+    back to within tol of the boundaries, from the finite side; finite ends are kept."""
+    from fenics_nonlinear_energies_amd.minimizers import repair_linesearch_interval
+
+    f = lambda a: a * a if -0.3 < a < 0.7 else float("inf")
+    a, b, evals, repaired = repair_linesearch_interval(f, -0.5, 2.0, center=0.0, tol=1e-6)
+    assert repaired and -0.3 < a <= -0.3 + 1e-6 and 0.7 - 1e-6 <= b < 0.7
+    assert evals > 2
+    a, b, _, repaired = repair_linesearch_interval(lambda t: t * t, -0.5, 2.0, tol=1e-6)
+    assert (a, b, repaired) == (-0.5, 2.0, False)
+    # a non-finite centre leaves the interval alone
+    a, b, _, repaired = repair_linesearch_interval(f, -0.5, 2.0, center=1.0, tol=1e-6)
+    assert (a, b, repaired) == (-0.5, 2.0, False)
