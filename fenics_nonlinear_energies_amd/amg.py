@@ -205,6 +205,7 @@ class SmoothedAggregationPC:
         self.levels: list[dict] = []
         self.setup_count = 0
         self.setup_time = 0.0
+        self.coarse_eig_fallbacks = 0  # set-ups whose coarsest operator was not positive definite
         self.symbolic_time = 0.0
         self._key = None
         self._A0 = None
@@ -379,6 +380,7 @@ class SmoothedAggregationPC:
         if info == 0:
             Ainv = np.tril(Ainv) + np.tril(Ainv, -1).T
         else:
+            self.coarse_eig_fallbacks += 1
             shift = 1e-12 * np.trace(Ad) / max(1, nc)
             w, V = np.linalg.eigh(Ad)
             w = np.where(w > shift, w, np.inf)

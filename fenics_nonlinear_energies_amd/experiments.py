@@ -63,6 +63,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     123-173): trust-region Newton with Steihaug-Toint CG subproblems (radius 0.5 at start) and
     a line search on [-0.5, 2] (tol 1e-1) inside the radius, multigrid rebuilt at every Newton
     iteration, KSP rtol 1e-1 capped at 30 iterations; stopping tolerances as for the others.
+    It also adds ``learned_trust_fp32``: the same settings from the learned guess in FP32.
     """
     from .problems.hyperelasticity import solve_he_load_steps
 
@@ -104,6 +105,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         trust = dict(use_trust_region=True, trust_radius_init=0.5, trust_subproblem_line_search=True,
                      linesearch_tol=1e-1)
         variants.insert(0, ("reference_trust_fp64", "previous", "fp64", False, None, 30, "stcg", trust))
+        # the same solver settings from the learned guess with FP32 Hessians
+        variants.append(("learned_trust_fp32", "learned", "fp32", False, None, 30, "stcg", trust))
     if with_reference:
         # the reference's "reference" profile on the GPU (hyperelasticity/jax_petsc/solver.py:35-46):
         # previous state, FP64, KSP capped at 30 iterations, PC kept and rebuilt at the cap
@@ -134,6 +137,8 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
             "learned_accepted": list(pred.accepted),
             "pc_setups": int(getattr(asm.ksp._amg, "setup_count", 0)) if asm.ksp._amg is not None else 0,
             "pc_setup_s": float(getattr(asm.ksp._amg, "setup_time", 0.0)) if asm.ksp._amg is not None else 0.0,
+            "pc_coarse_eig_fallbacks": int(getattr(asm.ksp._amg, "coarse_eig_fallbacks", 0))
+            if asm.ksp._amg is not None else 0,
         }
         log(f"{name}: nit={out['variants'][name]['newton_its']} wall={wall:.2f}s E={recs[-1]['energy']:.12e} "
             f"pc_setups={out['variants'][name]['pc_setups']} ({out['variants'][name]['pc_setup_s']:.1f}s)")
