@@ -1,0 +1,101 @@
+"""Host logic of the learned warm start (CPU tensors; the MLP itself is tested on the GPU).
+
+* free_node_table: the nodes whose components are all free, with their reordered free
+  indices in component order;
+* NodalPredictor: "previous" keeps the last state (None), "extrapolate" is
+  u_k + (u_k - u_{k-1}); the training target is the deviation of u_{k+1} from that
+  extrapolation on the feature nodes, scaled; the learned guess adds the network's
+  correction (times the target scale and the displacement scale) on those nodes only; the
+  energy guard keeps the extrapolation when the learned guess has the higher energy;
+* training_pairs: one row per node and per (k-1, k, k+1) triple, 3 dim + 1 features.
+"""
+
+import numpy as np
+import torch
+
+from fenics_nonlinear_energies_amd.experiments import free_node_table
+from fenics_nonlinear_energies_amd.warmstart import NodalPredictor, training_pairs
+
+
+def _setup(seed=0):
+    rng = np.random.default_rng(seed)
+    n_nodes, block = 12, 3
+    fixed = {0, 1, 2, 7, 20}  # dofs 0-2 (node 0), 7 (node 2, y), 20 (node 6, z)
+    freedofs = np.array([d for d in range(n_nodes * block) if d not in fixed], dtype=np.int64)
+    perm = rng.permutation(freedofs.size)
+    return rng, n_nodes, block, freedofs, perm
+
+
+def test_free_node_table_maps_nodes_to_reordered_free_indices():
+    _, n_nodes, block, freedofs, perm = _setup()
+    nodes, idx = free_node_table(freedofs, perm, block)
+    assert nodes.tolist() == [n for n in range(n_nodes) if n not in (0, 2, 6)]
+    reordered = freedofs[perm]  # reordered free index -> global dof
+    for i, nd in enumerate(nodes):
+        assert [int(reordered[j]) for j in idx[i]] == [block * nd + c for c in range(block)]
+
+
+def _predictor(mode, net=None):
+    rng, n_nodes, block, freedofs, perm = _setup()
+    nodes, idx = free_node_table(freedofs, perm, block)
+    coords = rng.random((nodes.size, block))
+    offset = torch.as_tensor(rng.random(freedofs.size))
+    pred = NodalPredictor(idx, coords, state_offset=offset, disp_scale=0.5, load_scale=2.0, net=net, mode=mode)
+    states = [torch.as_tensor(rng.random(freedofs.size)) for _ in range(4)]
+    history = [(0.5 * k, s) for k, s in enumerate(states)]
+    return pred, history, torch.as_tensor(idx)
+
+
+def test_previous_and_extrapolate_modes():
+    pred, history, _ = _predictor("previous")
+    assert pred(history, 2.0) is None
+    pred, history, _ = _predictor("extrapolate")
+    assert pred(history[:1], 0.5) is None  # one state: nothing to extrapolate from
+    g = pred(history, 2.0)
+    assert torch.allclose(g, 2.0 * history[-1][1] - history[-2][1], rtol=0, atol=1e-15)
+
+
+def test_target_is_the_scaled_deviation_from_extrapolation():
+    pred, history, idx = _predictor("extrapolate")
+    u_km1, u_k = history[1][1], history[2][1]
+    ext = 2.0 * u_k - u_km1
+    assert float(pred.target(u_k, u_km1, ext).abs().max()) == 0.0
+    u_next = history[3][1]
+    t = pred.target(u_k, u_km1, u_next)
+    assert t.shape == idx.shape
+    assert torch.allclose(t.double(), (u_next - ext)[idx] / 0.5, rtol=1e-6, atol=1e-6)
+
+
+class _ConstNet:
+    def __init__(self, value):
+        self.value = value
+
+    def forward(self, f):
+        return torch.full((f.shape[0], 3), self.value, dtype=torch.float32)
+
+
+def test_learned_guess_adds_the_correction_on_feature_nodes_and_the_guard_decides():
+    pred, history, idx = _predictor("learned", net=_ConstNet(0.25))
+    pred.targ_std = torch.tensor([2.0, 2.0, 2.0])
+    ext = 2.0 * history[-1][1] - history[-2][1]
+    g = pred(history, 2.0)
+    mask = torch.zeros_like(ext, dtype=torch.bool)
+    mask[idx.reshape(-1)] = True
+    assert torch.allclose(g[mask], ext[mask] + 0.5 * 0.25 * 2.0, rtol=0, atol=1e-12)
+    assert torch.equal(g[~mask], ext[~mask])
+    # guard: an energy that prefers the extrapolation rejects the learned guess
+    pred.energy_fn = lambda t: float((t - ext).abs().sum())
+    assert torch.equal(pred(history, 2.0), ext) and pred.accepted == [False]
+    pred.energy_fn = lambda t: -float((t - ext).abs().sum())
+    assert torch.allclose(pred(history, 2.0), g) and pred.accepted == [False, True]
+
+
+def test_training_pairs_shapes_and_normalisation():
+    pred, history, idx = _predictor("extrapolate")
+    x, t = training_pairs(pred, history)
+    n = idx.shape[0]
+    assert x.shape == (2 * n, 3 * 3 + 1) and t.shape == (2 * n, 3)
+    assert torch.allclose(x[:n, -1], torch.full((n,), history[2][0] / 2.0))  # next load / load_scale
+    xn, tn = pred.fit_normalization(x, t)
+    assert torch.allclose(xn.mean(dim=0), torch.zeros(x.shape[1]), atol=1e-5)
+    assert torch.allclose(tn.std(dim=0), torch.ones(3), atol=1e-5)
