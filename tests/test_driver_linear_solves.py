@@ -1,6 +1,6 @@
 """Host logic around the Newton driver: the preconditioner set-up policy and the
-Eisenstat-Walker forcing term of the linear solves, the retry schedule and the line-search
-interval repair.
+Eisenstat-Walker forcing term of the linear solves, the retry schedule, the line-search
+interval repair and the load-step loop.
 
 Set-up policy: by default before every solve; with pc_setup_on_ksp_cap only on the first
 solve and after a solve that hit ksp_max_it (core/petsc/scalar_problem_driver.py:150-202),
@@ -193,3 +193,35 @@ def test_line_search_interval_repair_bisects_to_the_finite_region():
     # a non-finite centre leaves the interval alone
     a, b, _, repaired = repair_linesearch_interval(f, -0.5, 2.0, center=1.0, tol=1e-6)
     assert (a, b, repaired) == (-0.5, 2.0, False)
+
+
+def test_load_step_loop_retries_and_stops():
+    """run_load_steps (core/petsc/load_step_driver.py:58-110): angle = step x rotation, the
+    next attempt only when should_retry says so and one is left, the record of the attempt
+    used, and the stop test after each step."""
+    from fenics_nonlinear_energies_amd.load_step_driver import LoadStepAttempt, run_load_steps
+
+    calls = []
+    attempts = [LoadStepAttempt("primary", (-0.5, 2.0), 1e-1, 30), LoadStepAttempt("repair", (-0.5, 1.0), 1e-2, 60)]
+
+    def solve(ctx, att):
+        calls.append((ctx.step, att.name))
+        ok = not (ctx.step == 2 and att.name == "primary")  # step 2 fails first time round
+        return {"ok": ok, "fun": float(ctx.step)}, 0.1
+
+    recs = run_load_steps(
+        start_step=1, num_steps=5, rotation_per_step=0.25,
+        prepare_step=lambda ctx: ctx.state.update(prepared=True),
+        build_attempts=lambda ctx: attempts,
+        solve_attempt=solve,
+        should_retry=lambda res, ctx: not res["ok"],
+        build_step_record=lambda ctx, res, t, att: {"step": ctx.step, "angle": ctx.angle, "attempt": att.name},
+        should_stop=lambda rec, res, ctx: ctx.step == 4,
+    )
+    assert calls == [(1, "primary"), (2, "primary"), (2, "repair"), (3, "primary"), (4, "primary")]
+    assert [r["attempt"] for r in recs] == ["primary", "repair", "primary", "primary"]
+    assert [r["angle"] for r in recs] == [0.25, 0.5, 0.75, 1.0]
+    with pytest.raises(ValueError):
+        run_load_steps(start_step=1, num_steps=1, rotation_per_step=1.0, prepare_step=lambda c: None,
+                       build_attempts=lambda c: [], solve_attempt=solve, should_retry=lambda r, c: False,
+                       build_step_record=lambda c, r, t, a: {})
