@@ -145,6 +145,28 @@ def _positions(M: sp.csr_matrix, rows: np.ndarray, cols: np.ndarray) -> np.ndarr
     return pos
 
 
+def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
+    """Dense inverse of the (symmetrised) coarsest operator for the V-cycle's coarse solve.
+
+    Cholesky (LAPACK potrf/potri) when the operator is positive definite; otherwise the
+    pseudo-inverse over its eigenvalues above 1e-12 mean(diag), which keeps the coarse
+    correction positive semi-definite when the Hessian is indefinite (the eigendecomposition
+    runs through torch's LAPACK, about twice as fast as numpy's here).  Returns (inverse,
+    whether the fallback was taken).
+    """
+    nc = Ad.shape[0]
+    c, info = sla.lapack.dpotrf(Ad, lower=1)
+    if info == 0:
+        Ainv, info = sla.lapack.dpotri(c, lower=1)  # lower triangle of the inverse
+    if info == 0:
+        return np.tril(Ainv) + np.tril(Ainv, -1).T, False
+    shift = 1e-12 * np.trace(Ad) / max(1, nc)
+    w, V = torch.linalg.eigh(torch.from_numpy(np.ascontiguousarray(Ad, dtype=np.float64)))
+    w, V = w.numpy(), V.numpy()
+    w = np.where(w > shift, w, np.inf)
+    return (V / w) @ V.T, True
+
+
 def _aggregate_sizes(n: int, agg: int) -> np.ndarray:
     na, rem = divmod(n, agg)
     return np.array([agg] * na + ([rem] if rem else []), dtype=np.int64)
@@ -370,21 +392,12 @@ class SmoothedAggregationPC:
             if lvl + 1 < len(self.levels):
                 self.levels[lvl + 1]["A"].data.copy_(Ac64.data)
             a_ptr, a_idx, a_dat = Ac64.indptr, Ac64.indices, Ac64.data
-        # coarsest level: dense inverse on the host (Cholesky; eigen-pseudo-inverse if singular)
+        # coarsest level: dense inverse on the host (Cholesky; positive-part pseudo-inverse if
+        # the operator is not positive definite)
         nc, cp = self._coarse_n, self._coarse_pat
         Ad = sp.csr_matrix((a_dat.double().cpu().numpy(), cp.indices, cp.indptr), shape=(nc, nc)).toarray()
-        Ad = 0.5 * (Ad + Ad.T)
-        c, info = sla.lapack.dpotrf(Ad, lower=1)
-        if info == 0:
-            Ainv, info = sla.lapack.dpotri(c, lower=1)  # lower triangle of the inverse
-        if info == 0:
-            Ainv = np.tril(Ainv) + np.tril(Ainv, -1).T
-        else:
-            self.coarse_eig_fallbacks += 1
-            shift = 1e-12 * np.trace(Ad) / max(1, nc)
-            w, V = np.linalg.eigh(Ad)
-            w = np.where(w > shift, w, np.inf)
-            Ainv = (V / w) @ V.T
+        Ainv, fallback = coarse_inverse(0.5 * (Ad + Ad.T))
+        self.coarse_eig_fallbacks += int(fallback)
         self.coarse_op.data.copy_(torch.as_tensor(np.ascontiguousarray(Ainv).ravel()))
         self.setup_count += 1
         self.setup_time += time.perf_counter() - t_setup

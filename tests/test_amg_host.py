@@ -48,3 +48,28 @@ def test_pattern_product_and_permutations():
 def _t_pattern(C):
     T = amg._pattern(C.T.tocsr())
     return T.indices, T.indptr
+
+
+def test_coarse_inverse_cholesky_and_indefinite_fallback():
+    """SPD coarse operator: the exact inverse (Cholesky).  Indefinite: the pseudo-inverse over
+    the positive eigenvalues (closed form from numpy's eigendecomposition), symmetric and
+    positive semi-definite, which the fallback flag reports."""
+    from fenics_nonlinear_energies_amd.amg import coarse_inverse
+
+    rng = np.random.default_rng(0)
+    n = 60
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.geomspace(1e-3, 10.0, n)
+    A = (q * lam) @ q.T
+    A = 0.5 * (A + A.T)
+    inv, fb = coarse_inverse(A)
+    assert not fb and np.allclose(inv @ A, np.eye(n), atol=1e-8)
+    lam2 = lam.copy()
+    lam2[:3] = [-2.0, -0.5, 0.0]
+    B = (q * lam2) @ q.T
+    B = 0.5 * (B + B.T)
+    inv2, fb2 = coarse_inverse(B)
+    keep = lam2 > 1e-12 * lam2.sum() / n
+    expect = (q[:, keep] / lam2[keep]) @ q[:, keep].T
+    assert fb2 and np.allclose(inv2, expect, rtol=1e-8, atol=1e-8 * np.abs(expect).max())
+    assert np.allclose(inv2, inv2.T, atol=1e-10) and np.linalg.eigvalsh(inv2).min() > -1e-8
