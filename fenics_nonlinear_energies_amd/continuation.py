@@ -13,10 +13,8 @@ learned warm start) and runs the Newton driver on device vectors.
 
 from __future__ import annotations
 
-import time
 from typing import Callable, Sequence
 
-import numpy as np
 import torch
 
 from .drivers import solve_newton
@@ -62,54 +60,51 @@ def run_lambda_continuation(
     ksp_max_it: int = 2000,
     pc_setup_on_ksp_cap: bool = True,
     pc_refresh_its: int | None = None,
+    retry_on_failure=False,
+    stop_on_fail: bool = False,
     newton_kwargs: dict | None = None,
 ) -> list[dict]:
     """Solve the reduced-strength problem for each lambda, warm-starting from the history.
 
     ``x0``: device vector (e.g. the elastic initial guess), overwritten with the last state.
-    ``predictor(history, lam)`` returns an initial guess tensor or None (then the previous
-    converged state is used).  ``history`` holds (lam, state tensor) pairs of converged steps.
-    With ``pc_setup_on_ksp_cap`` the preconditioner is kept across Newton iterations and steps
-    and rebuilt when a solve hits ``ksp_max_it`` or needs more than ``pc_refresh_its``.
+    ``predictor(history, lam)`` returns an initial guess tensor or None (then the state the
+    previous step left is used).  ``history`` holds (lam, state tensor) pairs of converged
+    steps.  With ``pc_setup_on_ksp_cap`` the preconditioner is kept across Newton iterations
+    and steps and rebuilt when a solve hits ``ksp_max_it`` or needs more than
+    ``pc_refresh_its``.  ``retry_on_failure`` is off by default, as in the reference's slope
+    solver (``solve_slope_stability_dof.py:378-381``).
     """
+    from .load_path import march
+
     p = asm.params
     phi_raw = float(p.get("phi_deg_raw", p["phi_deg"]))
     p["phi_deg_raw"] = phi_raw
     opts = dict(SLOPE_NEWTON)
     opts.update(newton_kwargs or {})
-    history: list[tuple[float, torch.Tensor]] = []
-    records = []
-    x = x0
-    for k, lam in enumerate(lambdas):
+    strength = {}
+
+    def apply(lam):
         c, phi = davis_b_reduction(float(p["c0"]), phi_raw, float(p["psi_deg"]), float(lam))
         asm.set_strength(c, phi)
-        guess = predictor(history, float(lam)) if (predictor is not None and history) else None
-        used_pred = guess is not None
-        if used_pred:
-            x.t.copy_(guess)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
-                           keep_pc=pc_setup_on_ksp_cap, pc_refresh_its=pc_refresh_its, **opts)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        rec = {
-            "step": k,
-            "lambda": float(lam),
-            "cohesion": float(c),
-            "phi_deg": float(phi),
-            "nit": int(res.get("nit", 0)),
-            "message": str(res.get("message", "")),
-            "energy": float(res.get("fun", np.nan)),
-            "linear_iters": int(res.get("linear_iters", 0)),
-            "u_max": float(x.t.abs().max().item()),
-            "time": dt,
-            "predictor": bool(used_pred),
-        }
-        records.append(rec)
-        history.append((float(lam), x.t.clone()))
-        if predictor is not None and hasattr(predictor, "accepted") and used_pred and predictor.accepted:
-            rec["learned_accepted"] = bool(predictor.accepted[-1])
+        strength.update(cohesion=float(c), phi_deg=float(phi))
+
+    def guess(history, lam):
+        return predictor(history, lam) if (predictor is not None and history) else None
+
+    def solve(vec, _lam):
+        return solve_newton(asm, vec, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
+                            keep_pc=pc_setup_on_ksp_cap, pc_refresh_its=pc_refresh_its,
+                            retry_on_failure=retry_on_failure, **opts)
+
+    def notify(rec, history):
+        rec.extra.update(strength)
+        rec.extra["u_max"] = float(x0.t.abs().max().item())
+        if predictor is not None and hasattr(predictor, "accepted") and rec.guessed and predictor.accepted:
+            rec.extra["learned_accepted"] = bool(predictor.accepted[-1])
         if on_step is not None:
-            on_step(rec, history)
-    return records
+            on_step(rec.as_dict("lambda"), history)
+
+    recs, _ = march(x0, list(enumerate(float(v) for v in lambdas)), apply=apply, solve=solve, guess=guess,
+                    stop=(lambda r: not r.converged) if stop_on_fail else None, on_step=notify,
+                    sync=torch.cuda.synchronize)
+    return [r.as_dict("lambda") for r in recs]

@@ -1,6 +1,5 @@
 // libfne_hip core: plan lifecycle, dispatch, energy finalisation, CSR SpMV, diagonal, dot.
 #include <cstring>
-#include <mutex>
 
 #include "fne_common.h"
 
@@ -105,13 +104,6 @@ int launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const VT* 
   FNE_LAUNCH_CHECK();
   return FNE_OK;
 }
-
-// dot-product scratch: one per device, allocated on first use (never on the hot path afterwards)
-struct DotScratch {
-  double2* partial = nullptr;
-};
-std::mutex g_dot_mu;
-DotScratch g_dot[64];
 
 int validate_desc(const fne_plan_desc* d) {
   if (!d) return FNE_EINVAL;
@@ -373,29 +365,21 @@ int fne_csr_diag(int32_t n_rows, const int32_t* indptr, const int32_t* indices, 
   return FNE_OK;
 }
 
-int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream) {
-  if (n < 0 || !out) return FNE_EINVAL;
-  int dev = 0;
-  FNE_HIP_CHECK(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) return FNE_EINVAL;
-  double2* partial;
-  {
-    std::lock_guard<std::mutex> lk(g_dot_mu);
-    if (!g_dot[dev].partial) {
-      if (hipMalloc(&g_dot[dev].partial, sizeof(double2) * kEnergyBlocks) != hipSuccess) {
-        fne_set_error("hipMalloc of dot scratch failed");
-        return FNE_ENOMEM;
-      }
-    }
-    partial = g_dot[dev].partial;
+int32_t fne_dot_work_len(void) { return kEnergyBlocks; }
+
+int fne_dot(int64_t n, const double* a, const double* b, double* out, void* work_, fne_stream_t stream) {
+  double2* work = static_cast<double2*>(work_);
+  if (n < 0 || !out || !work_ || (n > 0 && (!a || !b))) {
+    fne_set_error("fne_dot: negative length or null pointer");
+    return FNE_EINVAL;
   }
   int nb = int((n + kBlock - 1) / kBlock);
   if (nb > kEnergyBlocks) nb = kEnergyBlocks;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, n, a, b, partial);
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, n, a, b, work);
   FNE_LAUNCH_CHECK();
-  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kFinalizeThreads), 0, (hipStream_t)stream, partial,
-                     nb, out);
+  hipLaunchKernelGGL(finalize_pairs_kernel, dim3(1), dim3(kFinalizeThreads), 0, (hipStream_t)stream, work, nb,
+                     out);
   FNE_LAUNCH_CHECK();
   return FNE_OK;
 }

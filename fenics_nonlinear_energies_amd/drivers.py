@@ -16,7 +16,7 @@ import torch
 
 from . import native
 from .minimizers import newton
-from .repair import build_retry_attempts, needs_solver_repair
+from .repair import as_policy, primary_rung, solve_with_repair
 
 NEWTON_DEFAULTS = {
     "tolf": 1e-5,
@@ -232,29 +232,25 @@ def solve_newton(
             return ctx.assemble_and_solve(vec, rhs, sol, assemble=False)
 
     trust_ksp = bool(use_trust_region and str(assembler.ksp.ksp_type).lower() in {"stcg", "nash", "gltr"})
-    attempts = build_retry_attempts(
-        retry_on_failure=bool(retry_on_failure), linesearch_interval=ls, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it
-    )
+    policy = as_policy(retry_on_failure)
+    primary = primary_rung(ksp_rtol, ksp_max_it, ls)
+    rungs = [primary] if policy is None else policy.ladder(primary)
     dls = None
     if device_linesearch and getattr(assembler, "device", None) is not None and hasattr(assembler, "energy_device"):
         dls = getattr(assembler, "_device_golden", None)
         if dls is None:
             dls = assembler._device_golden = DeviceGoldenSearch(assembler)
-    x_initial = x.copy()
-    result = None
-    used = attempts[0]
-    t_solve = 0.0
-    for idx, (name, interval, rtol, max_it) in enumerate(attempts):
-        x_initial.copy(x)
-        ctx.reset(rtol, max_it)
-        used = (name, interval, rtol, max_it)
+    timing = {"t": 0.0}
+
+    def solve_once(rung):
+        ctx.reset(rung.ksp_rtol, rung.ksp_max_it)
         t0 = time.perf_counter()
-        result = newton(
+        res = newton(
             energy_fn=assembler.energy_fn,
             gradient_fn=assembler.gradient_fn,
             hessian_solve_fn=ctx.hessian_solve_fn,
             x=x,
-            linesearch_interval=interval,
+            linesearch_interval=rung.linesearch_interval,
             verbose=verbose,
             hessian_matvec_fn=lambda _x, vin, vout: assembler.A.mult(vin, vout),
             trust_subproblem_solve_fn=ctx.trust_subproblem_solve_fn if trust_ksp else None,
@@ -264,14 +260,19 @@ def solve_newton(
             device_linesearch=dls,
             **opts,
         )
-        t_solve = time.perf_counter() - t0
-        if needs_solver_repair(result) and idx + 1 < len(attempts):
-            continue
-        break
+        timing["t"] = time.perf_counter() - t0
+        return res
+
+    result, used, tries = solve_with_repair(
+        x, rungs, solve_once, policy.wants_repair if policy is not None else (lambda _r: False))
     result = dict(result)
     result["linear_iters"] = int(sum(ctx.iters))
     result["linear_iters_list"] = list(ctx.iters)
-    result["solve_time"] = float(t_solve)
-    result["attempt"] = used[0]
+    result["solve_time"] = float(timing["t"])
+    result["attempt"] = used.label
+    result["tries"] = int(tries)
+    result["ksp_rtol_used"] = float(used.ksp_rtol)
+    result["ksp_max_it_used"] = int(used.ksp_max_it)
+    result["linesearch_interval_used"] = list(used.linesearch_interval)
     result["linear_timing"] = list(ctx.records)
     return result

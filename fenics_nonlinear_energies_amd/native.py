@@ -128,7 +128,8 @@ _SIGNATURES = [
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
          C.c_void_p, C.c_void_p, C.c_void_p],
     ),
-    ("fne_dot", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fne_dot", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("fne_dot_work_len", C.c_int32, []),
     (
         "fne_color_patch_elements",
         C.c_int,

@@ -229,25 +229,27 @@ HE_NEWTON = dict(tolf=1e-4, tolg=1e-3, tolg_rel=1e-3, maxit=100, linesearch_inte
 def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * np.pi, *, num_steps=None,
                         predictor=None, on_step=None, ksp_rtol: float = 1e-1, ksp_max_it: int = 30,
                         pc_setup_on_ksp_cap: bool = False, pc_refresh_its: int | None = None,
-                        newton_kwargs: dict | None = None):
+                        retry_on_failure=True, stop_on_fail: bool = False, newton_kwargs: dict | None = None):
     """Rotate the right face in ``total_steps`` equal steps and solve each step with Newton.
 
-    Follows the reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-359``):
+    The reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-493``):
     angle_k = k * total_angle / total_steps, the rotated face is imposed through
-    ``update_dirichlet`` and the step starts from the previous converged state.
-    ``predictor(history, angle)`` may return another initial guess (device tensor of free
-    values); ``history`` holds (angle, free-state tensor) of the converged steps, starting with
-    the reference configuration at angle 0.  The reference rotates by 4 * 2 pi in total
-    (``solver.py:225``); the angle is a parameter here.  Preconditioner policy: set up at every
-    Newton iteration (the reference's), or with ``pc_setup_on_ksp_cap`` kept across iterations
-    and steps and rebuilt when a solve hits ``ksp_max_it`` or needs more than ``pc_refresh_its``
-    iterations.
+    ``update_dirichlet``, the step starts from the state the previous step left, and a failed
+    step (non-finite, Newton or linear iteration cap) is solved again from its start with the
+    tighter repair settings (``retry_on_failure``, on by default as in ``solve_HE_dof.py:165-170``;
+    a ``repair.RepairPolicy`` may be passed instead).  ``predictor(history, angle)`` may return
+    another initial guess (device tensor of free values); ``history`` holds (angle, state) of
+    the converged steps, starting with the reference configuration at angle 0.  The reference
+    rotates by 4 * 2 pi in total (``solver.py:225``); the angle is a parameter here.
+    Preconditioner policy: set up at every Newton iteration (the reference's), or with
+    ``pc_setup_on_ksp_cap`` kept across iterations and steps and rebuilt when a solve hits
+    ``ksp_max_it`` or needs more than ``pc_refresh_its`` iterations.  ``stop_on_fail`` ends the
+    path at the first step that does not converge.  Returns (records, state vector).
     """
-    import time
-
     import torch
 
     from ..drivers import solve_newton
+    from ..load_path import march, uniform_loads
 
     p = asm.params
     opts = dict(HE_NEWTON)
@@ -255,33 +257,21 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
     lay = asm.layout
     ref = np.asarray(p["u_0_ref"], dtype=np.float64)
     x = asm.create_vec(ref[p["freedofs"]][lay.perm])
-    history = [(0.0, x.t.clone())]
-    records = []
-    n_run = int(total_steps if num_steps is None else num_steps)
-    for k in range(1, n_run + 1):
-        angle = k * float(total_angle) / float(total_steps)
+
+    def apply(angle):
         asm.update_dirichlet(rotate_right_face_from_reference(ref, p["nodes2coord"], angle, p["right_nodes"]))
-        guess = predictor(history, angle) if predictor is not None else None
-        if guess is not None:
-            x.t.copy_(guess)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                           pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap,
-                           pc_refresh_its=pc_refresh_its, **opts)
-        torch.cuda.synchronize()
-        rec = {
-            "step": k,
-            "angle": angle,
-            "nit": int(res.get("nit", 0)),
-            "message": str(res.get("message", "")),
-            "energy": float(res.get("fun", np.nan)),
-            "linear_iters": int(res.get("linear_iters", 0)),
-            "time": time.perf_counter() - t0,
-            "predictor": guess is not None,
-        }
-        records.append(rec)
-        history.append((angle, x.t.clone()))
+
+    def solve(vec, _angle):
+        return solve_newton(asm, vec, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
+                            pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap,
+                            pc_refresh_its=pc_refresh_its, retry_on_failure=retry_on_failure, **opts)
+
+    def notify(rec, history):
         if on_step is not None:
-            on_step(rec, history)
-    return records, x
+            on_step(rec.as_dict("angle"), history)
+
+    recs, _ = march(x, uniform_loads(total_angle, total_steps, num_steps), apply=apply, solve=solve,
+                    guess=predictor, initial_history=[(0.0, x.t.clone())],
+                    stop=(lambda r: not r.converged) if stop_on_fail else None, on_step=notify,
+                    sync=torch.cuda.synchronize)
+    return [r.as_dict("angle") for r in recs], x

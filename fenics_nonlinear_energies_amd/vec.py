@@ -95,20 +95,31 @@ class DeviceVec:
         return math.sqrt(max(self.dot(self), 0.0))
 
 
-_SCRATCH: dict[int, torch.Tensor] = {}
+_SCRATCH: dict[int, tuple[torch.Tensor, torch.Tensor]] = {}
+
+
+def _dot_scratch(device: torch.device) -> tuple[torch.Tensor, torch.Tensor]:
+    """Per-device (result, partials) buffers of ``fne_dot``, allocated once by torch."""
+    dev = device.index if device.index is not None else torch.cuda.current_device()
+    buf = _SCRATCH.get(dev)
+    if buf is None:
+        n_pairs = int(native.lib().fne_dot_work_len())
+        buf = (torch.empty(8, dtype=torch.float64, device=device),
+               torch.empty(2 * n_pairs, dtype=torch.float64, device=device))
+        _SCRATCH[dev] = buf
+    return buf
 
 
 def dot_device(a: torch.Tensor, b: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-    """Deterministic compensated FP64 dot product, result left on the device."""
+    """Deterministic compensated FP64 dot product, result left on the device.
+
+    Uses one partials buffer per device: calls are ordered on the current stream."""
+    res, work = _dot_scratch(a.device)
     if out is None:
-        dev = a.device.index if a.device.index is not None else torch.cuda.current_device()
-        out = _SCRATCH.get(dev)
-        if out is None:
-            out = torch.empty(8, dtype=torch.float64, device=a.device)
-            _SCRATCH[dev] = out
-        out = out[:1]
+        out = res[:1]
     native.check(
-        native.lib().fne_dot(int(a.numel()), native.ptr(a), native.ptr(b), native.ptr(out), native.stream_ptr()),
+        native.lib().fne_dot(int(a.numel()), native.ptr(a), native.ptr(b), native.ptr(out), native.ptr(work),
+                             native.stream_ptr()),
         "fne_dot",
     )
     return out

@@ -290,8 +290,15 @@ int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, co
                     void* work, int64_t work_bytes, int32_t* its_out, int32_t* reason_out, double* rnorm_out,
                     fne_stream_t stream);
 
-/* Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i]. */
-int fne_dot(int64_t n, const double* a, const double* b, double* out, fne_stream_t stream);
+/*
+ * Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i] (the PETSc VecDot /
+ * VecNorm of the reference's Newton loop, core/petsc/minimizers.py:286, 329, 1003).
+ * work: caller-owned device scratch of fne_dot_work_len() (sum, compensation) pairs
+ * (16 bytes each), written by the call; calls in flight at the same time need distinct work
+ * buffers.  Allocates nothing.
+ */
+int32_t fne_dot_work_len(void);
+int fne_dot(int64_t n, const double* a, const double* b, double* out, void* work, fne_stream_t stream);
 
 /*
  * Host (CPU) helper for plan construction: greedy coloring of patch elements so that no two

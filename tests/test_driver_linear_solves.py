@@ -12,6 +12,7 @@ raised to gamma eta_{k-1}^alpha when that exceeds 0.1, capped by ksp_rtol and fl
 1e-8 (Eisenstat & Walker 1996, choice 2); the fixed policy always returns ksp_rtol.
 """
 
+import numpy as np
 import pytest
 
 from fenics_nonlinear_energies_amd.drivers import LinearSolveContext
@@ -151,32 +152,84 @@ def test_trust_radius_reaches_the_ksp():
     assert ksp.radii == [0.5]
 
 
-def test_retry_schedule():
-    """Primary attempt, plus with retry a repair attempt: rtol x 0.1 (floored at 1e-12), the
-    line-search interval's upper end capped at 1, the KSP cap doubled (core/benchmark/
-    repair.py:29-80); nothing to tighten -> no repair attempt."""
-    from fenics_nonlinear_energies_amd.repair import build_retry_attempts
+def test_repair_ladder():
+    """Primary rung, plus with repair a tighter rung: rtol x 0.1 (floored at 1e-12), the
+    line-search interval's upper end capped at 1, the KSP cap doubled (the reference's policy,
+    core/benchmark/repair.py:29-80); nothing to tighten -> no repair rung."""
+    from fenics_nonlinear_energies_amd.repair import Rung, RepairPolicy, primary_rung
 
-    base = dict(linesearch_interval=(-0.5, 2.0), ksp_rtol=1e-1, ksp_max_it=30)
-    assert build_retry_attempts(**base) == [("primary", (-0.5, 2.0), 1e-1, 30)]
-    att = build_retry_attempts(retry_on_failure=True, **base)
-    assert att[1][0] == "repair" and att[1][1] == (-0.5, 1.0) and att[1][3] == 60
-    assert att[1][2] == pytest.approx(1e-2)
-    assert build_retry_attempts(retry_on_nonfinite=False, retry_on_maxit=False, retry_on_failure=True, **base) == [
-        ("primary", (-0.5, 2.0), 1e-1, 30)]
-    tiny = build_retry_attempts(retry_on_failure=True, linesearch_interval=(-0.5, 0.5), ksp_rtol=1e-12, ksp_max_it=1)
-    assert tiny[1] == ("repair", (-0.5, 0.5), 1e-12, 2)
+    prim = primary_rung(1e-1, 30, (-0.5, 2.0))
+    assert prim == Rung("primary", 1e-1, 30, (-0.5, 2.0))
+    lad = RepairPolicy().ladder(prim)
+    assert len(lad) == 2 and lad[0] == prim
+    rep = lad[1]
+    assert rep.label == "repair" and rep.linesearch_interval == (-0.5, 1.0) and rep.ksp_max_it == 60
+    assert rep.ksp_rtol == pytest.approx(1e-2)
+    assert RepairPolicy(repair_nonfinite=False, repair_caps=False).ladder(prim) == [prim]
+    # rtol already at the floor, interval already inside [., 1]: only the cap moves (1 -> 2)
+    tiny = RepairPolicy().ladder(primary_rung(1e-12, 1, (-0.5, 0.5)))
+    assert tiny[1] == Rung("repair", 1e-12, 2, (-0.5, 0.5))
+    # an interval that the cap would empty keeps its upper end
+    assert RepairPolicy().ladder(primary_rung(1e-3, 10, (1.5, 2.0)))[1].linesearch_interval == (1.5, 2.0)
 
 
 def test_which_results_trigger_a_repair():
-    from fenics_nonlinear_energies_amd.repair import needs_solver_repair
+    from fenics_nonlinear_energies_amd.repair import RepairPolicy, classify_failure
 
-    assert needs_solver_repair({"fun": float("nan"), "message": "Converged"})
-    assert needs_solver_repair({"fun": 1.0, "message": "Non-finite energy after update at Newton iteration 3"})
-    assert needs_solver_repair({"fun": 1.0, "message": "Maximum number of iterations reached"})
-    assert not needs_solver_repair({"fun": 1.0, "message": "Maximum number of iterations reached"},
-                                   retry_on_maxit=False)
-    assert not needs_solver_repair({"fun": 1.0, "message": "Converged (energy, step, gradient)"})
+    pol = RepairPolicy()
+    assert classify_failure({"fun": float("nan"), "message": "Converged"}) == "nonfinite"
+    assert pol.wants_repair({"fun": float("nan"), "message": "Converged"})
+    assert pol.wants_repair({"fun": 1.0, "message": "Non-finite energy after update at Newton iteration 3"})
+    assert pol.wants_repair({"fun": 1.0, "message": "Maximum number of iterations reached"})
+    assert pol.wants_repair({"fun": 1.0, "message": "Linear solve failed at Newton iteration 2"})
+    assert not RepairPolicy(repair_caps=False).wants_repair({"fun": 1.0, "message": "Maximum number of iterations reached"})
+    assert not RepairPolicy(repair_nonfinite=False).wants_repair({"fun": float("inf"), "message": "x"})
+    assert not pol.wants_repair({"fun": 1.0, "message": "Converged (energy, step, gradient)"})
+    # the trust-region stops are not failures the reference repairs
+    assert not pol.wants_repair({"fun": 1.0, "message": "Trust-region rejected all candidate steps at Newton iteration 4"})
+    assert not pol.wants_repair({"fun": 1.0, "message": "Trust-region radius exhausted before full convergence"})
+
+
+class _HostVec:
+    """numpy-backed stand-in with the PETSc copy protocol and a torch view (``.t``)."""
+
+    def __init__(self, a):
+        import torch
+
+        self.t = torch.as_tensor(np.asarray(a, dtype=np.float64)).clone()
+
+    def copy(self, other=None):
+        if other is None:
+            return _HostVec(self.t.numpy())
+        other.t.copy_(self.t)
+        return other
+
+
+def test_solve_with_repair_restarts_from_the_entry_state():
+    """The repair rung starts from the state at entry, not from where the failed solve ended;
+    a result that needs no repair ends the ladder; the last rung's result is kept either way."""
+    from fenics_nonlinear_energies_amd.repair import RepairPolicy, primary_rung, solve_with_repair
+
+    pol = RepairPolicy()
+    rungs = pol.ladder(primary_rung(1e-1, 30, (-0.5, 2.0)))
+    x = _HostVec([1.0, 2.0])
+    seen = []
+
+    def fail_then_ok(rung):
+        seen.append((rung.label, x.t.tolist()))
+        x.t.add_(10.0)
+        return {"fun": 1.0, "message": "Maximum number of iterations reached" if rung.label == "primary"
+                else "Converged (energy, step, gradient)"}
+
+    res, used, tries = solve_with_repair(x, rungs, fail_then_ok, pol.wants_repair)
+    assert seen == [("primary", [1.0, 2.0]), ("repair", [1.0, 2.0])]
+    assert used.label == "repair" and tries == 2 and x.t.tolist() == [11.0, 12.0]
+    res, used, tries = solve_with_repair(x, rungs, lambda r: {"fun": 0.0, "message": "Converged"}, pol.wants_repair)
+    assert used.label == "primary" and tries == 1
+    res, used, tries = solve_with_repair(x, rungs, lambda r: {"fun": float("nan"), "message": ""}, pol.wants_repair)
+    assert used.label == "repair" and tries == 2 and res["message"] == ""
+    with pytest.raises(ValueError):
+        solve_with_repair(x, [], fail_then_ok, pol.wants_repair)
 
 
 def test_line_search_interval_repair_bisects_to_the_finite_region():
@@ -195,33 +248,59 @@ def test_line_search_interval_repair_bisects_to_the_finite_region():
     assert (a, b, repaired) == (-0.5, 2.0, False)
 
 
-def test_load_step_loop_retries_and_stops():
-    """run_load_steps (core/petsc/load_step_driver.py:58-110): angle = step x rotation, the
-    next attempt only when should_retry says so and one is left, the record of the attempt
-    used, and the stop test after each step."""
-    from fenics_nonlinear_energies_amd.load_step_driver import LoadStepAttempt, run_load_steps
+def test_load_path_history_guess_and_stop():
+    """march: loads applied in order, the guess hook sees only converged steps (an unconverged
+    step is recorded but not added to the history, and the next step starts from the state it
+    left), the stop test ends the path after the step it fires on."""
+    from fenics_nonlinear_energies_amd.load_path import march, uniform_loads
 
+    loads = uniform_loads(1.0, 4, 5)  # last one beyond the total: allowed, k * total / n
+    assert [k for k, _ in loads] == [1, 2, 3, 4, 5] and loads[1][1] == pytest.approx(0.5)
+    applied, starts, guesses = [], [], []
+    x = _HostVec([0.0])
+
+    def solve(vec, load):
+        starts.append(float(vec.t[0]))
+        vec.t.fill_(load)
+        return {"fun": load, "nit": 3, "message": "Maximum number of iterations reached" if load == 0.5
+                else "Converged (energy, step, gradient)", "attempt": "primary"}
+
+    def guess(history, load):
+        guesses.append([h[0] for h in history])
+        return None if load < 0.75 else history[-1][1] * 2.0
+
+    recs, hist = march(x, loads, apply=applied.append, solve=solve, guess=guess, initial_history=[(0.0, x.t.clone())],
+                       stop=lambda r: r.load >= 1.0)
+    assert applied == [0.25, 0.5, 0.75, 1.0]
+    assert guesses == [[0.0], [0.0, 0.25], [0.0, 0.25], [0.0, 0.25, 0.75]]
+    assert starts == [0.0, 0.25, 0.5, 1.5]  # step 3 guessed 2 x 0.25, step 4 guessed 2 x 0.75
+    assert [r.converged for r in recs] == [True, False, True, True]
+    assert [h[0] for h in hist] == [0.0, 0.25, 0.75, 1.0]
+    d = recs[1].as_dict("angle")
+    assert d["angle"] == 0.5 and d["nit"] == 3 and not d["converged"] and d["predictor"] is False
+    assert recs[2].as_dict()["predictor"] is True
+
+
+def test_load_path_repairs_a_failed_step():
+    """A load path whose solve repairs failures: the failed step is solved again from its own
+    start with the repair rung, and the record names the rung and the number of tries."""
+    from fenics_nonlinear_energies_amd.load_path import march, uniform_loads
+    from fenics_nonlinear_energies_amd.repair import RepairPolicy, primary_rung, solve_with_repair
+
+    pol = RepairPolicy()
     calls = []
-    attempts = [LoadStepAttempt("primary", (-0.5, 2.0), 1e-1, 30), LoadStepAttempt("repair", (-0.5, 1.0), 1e-2, 60)]
+    x = _HostVec([0.0])
 
-    def solve(ctx, att):
-        calls.append((ctx.step, att.name))
-        ok = not (ctx.step == 2 and att.name == "primary")  # step 2 fails first time round
-        return {"ok": ok, "fun": float(ctx.step)}, 0.1
+    def solve(vec, load):
+        def once(rung):
+            calls.append((load, rung.label, float(vec.t[0])))
+            vec.t.add_(1.0)
+            bad = load == 2.0 and rung.label == "primary"
+            return {"fun": float("nan") if bad else load, "message": "Converged"}
+        res, rung, tries = solve_with_repair(vec, pol.ladder(primary_rung(1e-1, 30, (-0.5, 2.0))), once, pol.wants_repair)
+        return dict(res, attempt=rung.label, tries=tries)
 
-    recs = run_load_steps(
-        start_step=1, num_steps=5, rotation_per_step=0.25,
-        prepare_step=lambda ctx: ctx.state.update(prepared=True),
-        build_attempts=lambda ctx: attempts,
-        solve_attempt=solve,
-        should_retry=lambda res, ctx: not res["ok"],
-        build_step_record=lambda ctx, res, t, att: {"step": ctx.step, "angle": ctx.angle, "attempt": att.name},
-        should_stop=lambda rec, res, ctx: ctx.step == 4,
-    )
-    assert calls == [(1, "primary"), (2, "primary"), (2, "repair"), (3, "primary"), (4, "primary")]
-    assert [r["attempt"] for r in recs] == ["primary", "repair", "primary", "primary"]
-    assert [r["angle"] for r in recs] == [0.25, 0.5, 0.75, 1.0]
-    with pytest.raises(ValueError):
-        run_load_steps(start_step=1, num_steps=1, rotation_per_step=1.0, prepare_step=lambda c: None,
-                       build_attempts=lambda c: [], solve_attempt=solve, should_retry=lambda r, c: False,
-                       build_step_record=lambda c, r, t, a: {})
+    recs, _ = march(x, uniform_loads(3.0, 3), apply=lambda v: None, solve=solve)
+    assert calls == [(1.0, "primary", 0.0), (2.0, "primary", 1.0), (2.0, "repair", 1.0), (3.0, "primary", 2.0)]
+    assert [r.as_dict()["attempt"] for r in recs] == ["primary", "repair", "primary"]
+    assert [r.as_dict()["tries"] for r in recs] == [1, 2, 1]

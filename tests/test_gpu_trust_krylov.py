@@ -233,9 +233,15 @@ def test_trust_region_load_steps_he_level1_published_energy(require_gpu):
     pub = [r for r in fixtures.published_energies()["hyperelasticity"] if r["level"] == 1][0]["jax_petsc_element"]
     nit = sum(r["nit"] for r in recs)
     print("HE L1 trust region:", len(recs), "steps", nit, "Newton its", recs[-1]["energy"], "published", pub, flush=True)
+    amg = getattr(asm.ksp, "_amg", None)
+    fallbacks = None if amg is None else int(amg.coarse_eig_fallbacks)
+    failed = [(r["step"], r["message"], r["nit"], r["energy"], r.get("attempt")) for r in recs
+              if "onverged" not in r["message"]]
+    print("coarse eigendecomposition fallbacks:", fallbacks, "unconverged steps:", failed, flush=True)
     assert len(recs) == 24
     assert all(math.isfinite(r["energy"]) for r in recs)
-    assert all("onverged" in r["message"] for r in recs), [r["message"] for r in recs]
+    assert not failed, f"unconverged steps (step, message, nit, energy, attempt): {failed}; " \
+                       f"coarse eigendecomposition fallbacks: {fallbacks}"
     assert abs(recs[-1]["energy"] - pub) <= 2e-4 * abs(pub)
     asm.cleanup()
 

@@ -204,3 +204,86 @@ def test_line_search_branch_equals_the_oracle_newton():
     assert res["nit"] == ref["nit"]
     assert abs(res["fun"] - ref["fun"]) <= 1e-13 * abs(ref["fun"])
     assert np.abs(x.a - ref["x"]).max() <= 1e-11 * np.abs(ref["x"]).max()
+
+
+def _plaplace_callbacks(nx, seed):
+    from fenics_nonlinear_energies_amd.problems.plaplace import build_plaplace_problem
+    from oracle import assembly
+
+    params, _, _ = build_plaplace_problem(nx)
+    n = params["freedofs"].size
+    u0 = np.random.default_rng(seed).random(n)
+    E = lambda u: assembly.assemble("plaplace", params, u, want=("e",))["e"]
+    G = lambda u: assembly.assemble("plaplace", params, u, want=("g",))["g"]
+    H = lambda u: assembly.assemble("plaplace", params, u, want=("h",))["h"]
+    return u0, E, G, H
+
+
+def _vec_callbacks(E, G, H):
+    def gradient_fn(v, g):
+        g.a[:] = G(v.a)
+
+    def matvec(x, vin, vout):
+        vout.a[:] = H(x.a) @ vin.a
+
+    return (lambda v: E(v.a)), gradient_fn, matvec
+
+
+def test_trust_region_stcg_branch_equals_the_oracle_trust_newton():
+    """The trust-region branch with a trust KSP (Jacobi Steihaug CG) and the line search inside
+    the radius, against the oracle's restatement (oracle.solvers.newton_trust) on p-Laplace:
+    the same iterations, radii and energy (both evaluate the same points in the same order)."""
+    u0, E, G, H = _plaplace_callbacks(12, 5)
+    kw = dict(tolf=1e-5, tolg=1e-3, tolg_rel=1e-3, tolx_rel=1e-3, tolx_abs=1e-10, maxit=100)
+
+    def oracle_ts(u, rhs, radius):
+        s, its, _ = solvers.stcg(H(u), rhs, radius=radius, rtol=1e-2)
+        return s, its
+
+    ref = solvers.newton_trust(E, G, lambda u, v: H(u) @ v, u0, trust_solve=oracle_ts, line_search=True, radius=0.5,
+                               ls_tol=1e-1, **kw)
+
+    def trust_solve(x, rhs, sol, radius):
+        s, its = oracle_ts(x.a, rhs.a, radius)
+        sol.a[:] = s
+        return its
+
+    energy_fn, gradient_fn, matvec = _vec_callbacks(E, G, H)
+    x = NumpyVec(u0)
+    res = newton(energy_fn=energy_fn, gradient_fn=gradient_fn, hessian_solve_fn=None, x=x, hessian_matvec_fn=matvec,
+                 trust_subproblem_solve_fn=trust_solve, trust_region=True, trust_radius_init=0.5,
+                 trust_subproblem_line_search=True, linesearch_tol=1e-1, require_all_convergence=True,
+                 save_history=True, **kw)
+    assert "converged" in ref["message"].lower(), ref["message"]
+    assert res["message"] == ref["message"] and res["nit"] == ref["nit"]
+    assert [h["trust_radius"] for h in res["history"]] == ref["radii"]
+    assert abs(res["fun"] - ref["fun"]) <= 1e-13 * abs(ref["fun"])
+    assert np.abs(x.a - ref["x"]).max() <= 1e-11 * np.abs(ref["x"]).max()
+
+
+def test_trust_region_model_branch_equals_the_oracle_trust_newton():
+    """The 1-D / 2-D model branch (no trust KSP: Newton direction by Jacobi CG, the gradient as
+    the second direction) against the oracle restatement on p-Laplace."""
+    u0, E, G, H = _plaplace_callbacks(10, 6)
+    kw = dict(tolf=1e-6, tolg=1e-4, tolx_rel=1e-6, tolx_abs=1e-10, maxit=100)
+
+    def oracle_solve(u, rhs):
+        s, its, _ = solvers.cg(H(u), rhs, rtol=1e-3)
+        return s, its
+
+    ref = solvers.newton_trust(E, G, lambda u, v: H(u) @ v, u0, solve=oracle_solve, radius=0.3, ls_tol=1e-3, **kw)
+
+    def hess_solve(x, rhs, sol):
+        s, its = oracle_solve(x.a, rhs.a)
+        sol.a[:] = s
+        return its
+
+    energy_fn, gradient_fn, matvec = _vec_callbacks(E, G, H)
+    x = NumpyVec(u0)
+    res = newton(energy_fn=energy_fn, gradient_fn=gradient_fn, hessian_solve_fn=hess_solve, x=x,
+                 hessian_matvec_fn=matvec, trust_region=True, trust_radius_init=0.3, linesearch_tol=1e-3,
+                 require_all_convergence=True, save_history=True, **kw)
+    assert "converged" in ref["message"].lower(), ref["message"]
+    assert res["message"] == ref["message"] and res["nit"] == ref["nit"]
+    assert [h["trust_radius"] for h in res["history"]] == ref["radii"]
+    assert abs(res["fun"] - ref["fun"]) <= 1e-13 * abs(ref["fun"])

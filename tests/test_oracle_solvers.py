@@ -12,8 +12,18 @@
   interior one; negative curvature ends on the boundary along p.
 * golden_section: the minimiser of a parabola within tol, with the evaluation count the
   bracket's 1/gamma contraction fixes.
-* the trust-region model solvers of the Newton driver (minimizers.solve_trust_1d/2d):
-  closed forms.
+* the trust-region model solvers of the Newton driver (minimizers.solve_trust_1d/2d) and of
+  the oracle (solvers.trust_1d/2d): closed forms.
+* newton_golden: a quadratic with exact directions converges at the second iteration (the
+  first step lands within ls_tol of the minimiser); on x^4/4 the Newton step is -x/3 and the line minimum (alpha = 3)
+  lies beyond the interval, so alpha = 2 exactly and x_k = x_0 / 3^k; the tolg / tolg_rel
+  targets stop that sequence at the iteration its closed form gives.
+* newton_trust: on |x - c|^2 / 2 with a boundary-clipped exact subproblem every step is
+  accepted with rho = 1 and lands on the boundary, so the radius grows by 1.5 per iteration
+  until the full step fits (closed-form count and radii); a subproblem that returns an ascent
+  step is rejected, the radius halves 7 times, and the reference's two exhaustion messages
+  follow from the step-size test; its 2-D model branch takes the exact Newton step on a
+  quadratic.
 """
 
 import math
@@ -186,7 +196,8 @@ def test_golden_section_parabola_and_evaluation_count():
         assert n == len(seen) and n in (4 + m, 5 + m)
 
 
-def test_trust_1d_closed_form():
+@pytest.mark.parametrize("solve_trust_1d", [solve_trust_1d, solvers.trust_1d], ids=["product", "oracle"])
+def test_trust_1d_closed_form(solve_trust_1d):
     assert solve_trust_1d(1.0, 2.0, 10.0) == -0.5
     assert solve_trust_1d(1.0, 2.0, 0.2) == -0.2
     assert solve_trust_1d(-1.0, -3.0, 0.4) == 0.4  # negative curvature: to the boundary downhill
@@ -195,7 +206,8 @@ def test_trust_1d_closed_form():
     assert solve_trust_1d(1.0, 2.0, 0.0) == 0.0
 
 
-def test_trust_2d_interior_and_boundary():
+@pytest.mark.parametrize("solve_trust_2d", [solve_trust_2d, solvers.trust_2d], ids=["product", "oracle"])
+def test_trust_2d_interior_and_boundary(solve_trust_2d):
     mm = np.array([[3.0, 1.0], [1.0, 2.0]])
     rhs = np.array([0.5, -0.25])
     s, q = solve_trust_2d(rhs, mm, 10.0)
@@ -210,3 +222,79 @@ def test_trust_2d_interior_and_boundary():
     pts = delta * np.stack([np.cos(th), np.sin(th)], axis=1)
     q_exact = min(float(rhs @ p + 0.5 * p @ mm @ p) for p in pts)
     assert q_exact <= q <= q_exact + 4.0 * delta**2 * (math.pi / 181) ** 2 + 2.0 * delta * np.linalg.norm(rhs) * (math.pi / 181) ** 2
+
+
+def test_newton_golden_quadratic_converges_at_the_second_iteration():
+    A, b = _spd(6, 50.0, 11)
+    A = A.toarray()
+    xs = np.linalg.solve(A, b)
+    out = solvers.newton_golden(lambda x: _q(A, b, x), lambda x: A @ x - b, lambda x, r: (np.linalg.solve(A, r), 1),
+                                np.zeros(6), tolf=1e-12, tolg=1e-6, tolx_rel=1e-4, ls_tol=1e-7)
+    assert out["nit"] == 2 and out["message"] == "Converged (energy, step, gradient)"
+    # each search pins alpha to 1 within max(ls_tol, the sqrt(eps) flatness of the energy near its minimum)
+    assert np.linalg.norm(out["x"] - xs) <= 1e-7 * np.linalg.norm(xs)
+    one = solvers.newton_golden(lambda x: _q(A, b, x), lambda x: A @ x - b, lambda x, r: (np.linalg.solve(A, r), 1),
+                                np.zeros(6), tolf=0.0, tolg=0.0, tolx_rel=0.0, tolx_abs=0.0, maxit=1, ls_tol=1e-7)
+    assert one["message"] == "Maximum number of iterations reached"
+    assert np.linalg.norm(one["x"] - xs) <= 1e-7 * np.linalg.norm(xs)  # |1 - alpha| <= ls_tol
+
+
+def _quartic():
+    return (lambda x: float(np.sum(x**4) / 4.0), lambda x: x**3, lambda x, r: (r / (3.0 * x * x), 1))
+
+
+def test_newton_golden_quartic_line_search_is_clipped_at_two():
+    f, g, solve = _quartic()
+    out = solvers.newton_golden(f, g, solve, np.array([1.0, -2.0]), tolf=0.0, tolg=0.0, tolx_rel=0.0, tolx_abs=0.0,
+                                maxit=5, ls_tol=1e-3)
+    assert out["nit"] == 5 and out["message"] == "Maximum number of iterations reached"
+    assert np.allclose(out["x"], np.array([1.0, -2.0]) / 3.0**5, rtol=1e-14, atol=0.0)
+
+
+def test_newton_golden_gradient_targets():
+    f, g, solve = _quartic()
+    # |g_k| = 3^(-3k) at x_0 = 1: below 1e-3 first at k = 3, below max(1e-3, 0.01 |g_0|) at k = 2
+    out = solvers.newton_golden(f, g, solve, np.array([1.0]), tolg=1e-3, require_all=False, maxit=20)
+    assert (out["nit"], out["message"]) == (3, "Gradient norm converged")
+    out = solvers.newton_golden(f, g, solve, np.array([1.0]), tolg=1e-3, tolg_rel=0.01, require_all=False, maxit=20)
+    assert (out["nit"], out["message"]) == (2, "Gradient norm converged")
+
+
+def _clipped_newton(c):
+    def trust_solve(x, rhs, radius):  # exact subproblem of |x - c|^2 / 2: the Newton step, clipped
+        n = float(np.linalg.norm(rhs))
+        return (rhs if n <= radius else rhs * (radius / n)), 1
+    return (lambda x: 0.5 * float((x - c) @ (x - c)), lambda x: x - c, lambda x, v: v.copy(), trust_solve)
+
+
+def test_newton_trust_boundary_steps_expand_the_radius():
+    c = np.array([0.6, -0.8])  # |c| = 1 from x_0 = 0
+    f, g, hv, ts = _clipped_newton(c)
+    out = solvers.newton_trust(f, g, hv, np.zeros(2), trust_solve=ts, radius=0.1, tolg=1e-10, require_all=False)
+    # travelled 0.1 (1.5^k - 1) / 0.5 after k boundary steps: 0.1, 0.25, 0.475, 0.8125, then 0.1875 < 0.50625 fits
+    assert (out["nit"], out["message"]) == (5, "Gradient norm converged")
+    assert np.allclose(out["radii"], [0.15, 0.225, 0.3375, 0.50625, 0.50625], rtol=1e-14, atol=0)
+    assert np.linalg.norm(out["x"] - c) <= 1e-15
+
+
+def test_newton_trust_rejections_shrink_and_stop():
+    f, g, hv, _ = _clipped_newton(np.array([1.0, 0.0]))
+    uphill = lambda x, rhs, radius: (-rhs * (radius / np.linalg.norm(rhs)), 1)  # pred < 0: every trial rejected
+    out = solvers.newton_trust(f, g, hv, np.zeros(2), trust_solve=uphill, radius=1.0)
+    assert out["nit"] == 1 and out["message"] == "Trust-region rejected all candidate steps at Newton iteration 1"
+    assert out["radii"] == [0.5**7] and np.all(out["x"] == 0.0) and len(out["linear_iters"]) == 7
+    # the last trial's step (1e-9 / 64) is below tolx_abs: exhausted, gradient still large
+    out = solvers.newton_trust(f, g, hv, np.zeros(2), trust_solve=uphill, radius=1e-9, tolx_abs=1e-10)
+    assert out["message"] == "Trust-region radius exhausted before full convergence"
+    out = solvers.newton_trust(f, g, hv, np.zeros(2), trust_solve=uphill, radius=1e-9, tolx_abs=1e-10, tolg=10.0)
+    assert out["message"] == "Trust-region step converged"
+
+
+def test_newton_trust_model_branch_takes_the_newton_step():
+    A, b = _spd(5, 20.0, 4)
+    A = A.toarray()
+    xs = np.linalg.solve(A, b)
+    out = solvers.newton_trust(lambda x: _q(A, b, x), lambda x: A @ x - b, lambda x, v: A @ v, np.zeros(5),
+                               solve=lambda x, r: (np.linalg.solve(A, r), 1), radius=100.0, maxit=1, ls_tol=1e-8,
+                               tolf=0.0, tolg=0.0, tolx_rel=0.0, tolx_abs=0.0)
+    assert np.linalg.norm(out["x"] - xs) <= 1e-8 * np.linalg.norm(xs)
