@@ -7,6 +7,13 @@ assembly: energy + gradient + CSR Hessian from one fused staged-patch kernel plu
 finalisation kernel (Hessian values FP32, element math, energy and gradient FP64).  Inputs are
 resident in HBM before timing starts.
 
+Beside the headline the line carries: ``headline_fp64`` (the same workload with FP64 Hessian
+values), ``secondary`` (assembly and SpMV of the P4 Mohr-Coulomb slope at 1.57M DOFs and of the
+Neo-Hookean tet beams at 61,440 and 491,520 elements), ``time_to_tol`` (the metric's second
+half: Newton time to tolerance on the Neo-Hookean load path, the reference's solver profile in
+FP64 against FP32 Hessians from the learned warm start, network trained in the same run),
+``roofline`` and ``cpu_baseline``.
+
 Prints one JSON line (see the field comments below).  ``--gpus N`` with N > 1 exits: this
 build targets one GPU.
 """
@@ -37,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-sample-n", type=int, default=256)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--no-secondary", action="store_true", help="skip the P4 slope and tet beam assembly lines")
+    ap.add_argument("--no-fp64-line", action="store_true", help="skip the FP64-Hessian line of the headline workload")
+    ap.add_argument("--no-time-to-tol", action="store_true",
+                    help="skip the Neo-Hookean load-path time to tolerance (plain FP64 vs mixed + learned)")
     ap.add_argument(
         "--copy-calib-mib", type=int, default=0,
         help="before warm-up, run 3 device copies of this many MiB (counter calibration reference)",
@@ -126,8 +136,68 @@ def _time_ms(fn, reps=20):
     return a.elapsed_time(b) / reps
 
 
+def headline_fp64(n_cells: int, steps: int = 50) -> dict:
+    """The headline workload with FP64 Hessian values (the reference's precision): elem/s and the
+    HBM fraction of the same fused kernel, timed like the headline."""
+    import numpy as np
+    import torch
+
+    from fenics_nonlinear_energies_amd.problems.ginzburg_landau import (
+        GLReorderedElementAssembler,
+        build_gl_problem,
+        sine_noise_init,
+    )
+
+    params, adj = build_gl_problem(n_cells)
+    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision="fp64")
+    x = asm.create_vec(sine_noise_init(params, 0)[asm.layout.perm])
+    g = x.duplicate()
+    ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t), reps=steps)
+    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
+    b_asm = ne * (4 * 3 + 8 * 7) + 8 * params["nodes"].shape[0] + 8 * nf + 8 * nnz
+    asm.cleanup()
+    torch.cuda.synchronize()
+    return {"workload": f"Ginzburg-Landau {n_cells}x{n_cells} P1, fused energy + gradient + Hessian, Hessian fp64",
+            "elem_per_s": ne / (ms * 1e-3), "assembly_ms": ms, "algorithmic_bytes": int(b_asm),
+            "achieved_GBs": b_asm / (ms * 1e-3) / 1e9, "frac": b_asm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def time_to_tol_beam() -> dict:
+    """Newton time to tolerance on the Neo-Hookean beam load path (160x8x8 hexes x 6 tets, 38,637
+    free DOFs, 24 rotation steps to 4 pi, every step to ||g|| <= 2e-4): the reference's solver
+    profile in FP64 from the previous state against FP32 Hessians from the learned warm start.
+    The network (10-128x4-3, BF16 MFMA kernels) is trained in this run on 8 seeded runs of the
+    80x4x4 beam (E ~ U[1e8, 4e8], nu ~ U[0.25, 0.4]); the evaluation material is unseen."""
+    import numpy as np
+
+    from fenics_nonlinear_energies_amd.experiments import he_warmstart_study
+
+    r = np.random.default_rng(0)
+    mats = [(float(r.uniform(1e8, 4e8)), float(r.uniform(0.25, 0.4))) for _ in range(8)]
+    nk = dict(tolf=0.0, tolg=2e-4, tolg_rel=0.0, tolx_rel=0.0, tolx_abs=0.0, maxit=100, require_all_convergence=False)
+    t0 = time.perf_counter()
+    out = he_warmstart_study(train_materials=mats, eval_material=(2e8, 0.3), newton_kwargs=nk, log=lambda *a: None)
+    total = time.perf_counter() - t0
+    v = out["variants"]
+    ref, lrn = v["reference_fp64"], v["learned_fp32"]
+    return {
+        "workload": "Neo-Hookean beam 160x8x8 hexes x 6 tets, 24 rotation steps to 4 pi, Newton to ||g|| <= 2e-4",
+        "plain_fp64_s": ref["wall_s"],
+        "mixed_learned_s": lrn["wall_s"],
+        "speedup": ref["wall_s"] / lrn["wall_s"],
+        "final_energy_rel_diff": abs(lrn["final_energy"] - ref["final_energy"]) / abs(ref["final_energy"]),
+        "variants": {k: {"wall_s": x["wall_s"], "newton_its": x["newton_its"], "linear_its": x["linear_its"],
+                         "all_converged": x["all_converged"], "final_energy": x["final_energy"]}
+                     for k, x in v.items()},
+        "training": {"samples": out["train_samples"], "loss_first": out["loss_first"], "loss_last": out["loss_last"],
+                     "train_s": out["train_time_s"], "data_runs_s": out["data_gen_time_s"]},
+        "study_total_s": total,
+    }
+
+
 def secondary_workloads() -> list[dict]:
-    """Fused Newton-step assembly and SpMV on the P4 slope (C5 size) and the tet beam (C3 size)."""
+    """Fused Newton-step assembly and SpMV on the P4 slope (C5 size) and the tet beams (C3 and C4
+    sizes)."""
     import numpy as np
 
     from fenics_nonlinear_energies_amd.problems import slope_stability as S
@@ -155,18 +225,21 @@ def secondary_workloads() -> list[dict]:
                 "mfma_TFLOPs": ne * 112.9e3 / (ms * 1e-3) / 1e12, "matfree_hvp_ms": mf})
     asm.cleanup()
     del asm
-    p, adj, u0 = build_beam_problem(160, 8, 8)
-    asm = HEReorderedElementAssembler(p, None, adj, hessian_precision="fp32")
-    x = asm.create_vec((u0 + 1e-4 * np.random.default_rng(0).standard_normal(u0.size))[asm.layout.perm])
-    g, y = x.duplicate(), x.duplicate()
-    ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t))
-    sp = _time_ms(lambda: asm.A.mult(x, y))
-    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
-    b_asm = ne * (4 * 12 + 8 * 13) + 8 * asm.dirichlet_full.size + 8 * nf + 4 * nnz
-    out.append({"workload": "Neo-Hookean beam 160x8x8 hexes x 6 tets", "n_elem": ne, "n_free": nf, "nnz": int(nnz),
-                "assembly_ms": ms, "elem_per_s": ne / (ms * 1e-3), "assembly_GBs": b_asm / (ms * 1e-3) / 1e9,
-                "spmv_ms": sp, "spmv_GBs": (8 * nnz + 20 * nf) / (sp * 1e-3) / 1e9})
-    asm.cleanup()
+    for dims in ((160, 8, 8), (320, 16, 16)):
+        p, adj, u0 = build_beam_problem(*dims)
+        asm = HEReorderedElementAssembler(p, None, adj, hessian_precision="fp32")
+        x = asm.create_vec((u0 + 1e-4 * np.random.default_rng(0).standard_normal(u0.size))[asm.layout.perm])
+        g, y = x.duplicate(), x.duplicate()
+        ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t))
+        sp = _time_ms(lambda: asm.A.mult(x, y))
+        ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
+        b_asm = ne * (4 * 12 + 8 * 13) + 8 * asm.dirichlet_full.size + 8 * nf + 4 * nnz
+        out.append({"workload": "Neo-Hookean beam %dx%dx%d hexes x 6 tets" % dims, "n_elem": ne, "n_free": nf,
+                    "nnz": int(nnz), "assembly_ms": ms, "elem_per_s": ne / (ms * 1e-3),
+                    "assembly_GBs": b_asm / (ms * 1e-3) / 1e9, "assembly_frac": b_asm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "spmv_ms": sp, "spmv_GBs": (8 * nnz + 20 * nf) / (sp * 1e-3) / 1e9})
+        asm.cleanup()
+        del asm
     return out
 
 
@@ -290,8 +363,12 @@ def main():
         },
     }
     asm.cleanup()
+    if not args.no_fp64_line and args.precision == "fp32":
+        out["headline_fp64"] = headline_fp64(args.n)
     if not args.no_secondary:
         out["secondary"] = secondary_workloads()
+    if not args.no_time_to_tol:
+        out["time_to_tol"] = time_to_tol_beam()
     if not args.no_cpu_baseline:
         single = cpu_baseline(args.cpu_sample_n, args.cpu_budget_s)
         omp = cpu_baseline_omp(args.n, args.cpu_budget_s)

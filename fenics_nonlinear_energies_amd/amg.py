@@ -48,10 +48,9 @@ from . import native
 
 
 def _lanes_for(avg: float) -> int:
-    lpr = 1
-    while lpr < 64 and 3 * lpr < avg:
-        lpr *= 2
-    return lpr
+    from .linalg import lanes_per_row
+
+    return lanes_per_row(avg)
 
 
 # fne_vcycle_op operations
@@ -149,10 +148,10 @@ def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
     """Dense inverse of the (symmetrised) coarsest operator for the V-cycle's coarse solve.
 
     Cholesky (LAPACK potrf/potri) when the operator is positive definite; otherwise the
-    pseudo-inverse over its eigenvalues above 1e-12 mean(diag), which keeps the coarse
-    correction positive semi-definite when the Hessian is indefinite (the eigendecomposition
-    runs through torch's LAPACK, about twice as fast as numpy's here).  Returns (inverse,
-    whether the fallback was taken).
+    pseudo-inverse over its eigenvalues above 1e-12 max|eigenvalue| (a positive cutoff whatever
+    the sign of the trace), which keeps the coarse correction positive semi-definite when the
+    Hessian is indefinite (the eigendecomposition runs through torch's LAPACK, about twice as
+    fast as numpy's here).  Returns (inverse, whether the fallback was taken).
     """
     nc = Ad.shape[0]
     c, info = sla.lapack.dpotrf(Ad, lower=1)
@@ -160,11 +159,29 @@ def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
         Ainv, info = sla.lapack.dpotri(c, lower=1)  # lower triangle of the inverse
     if info == 0:
         return np.tril(Ainv) + np.tril(Ainv, -1).T, False
-    shift = 1e-12 * np.trace(Ad) / max(1, nc)
     w, V = torch.linalg.eigh(torch.from_numpy(np.ascontiguousarray(Ad, dtype=np.float64)))
     w, V = w.numpy(), V.numpy()
-    w = np.where(w > shift, w, np.inf)
+    cut = 1e-12 * float(np.abs(w).max()) if w.size else 0.0
+    w = np.where(w > cut, w, np.inf)
     return (V / w) @ V.T, True
+
+
+def _injection_rows(P: sp.csr_matrix) -> np.ndarray:
+    """For each column j of an interpolation P, a row whose only entry is P[i, j] = 1 (the fine
+    DOF that coincides with coarse DOF j); raises if a column has none."""
+    P = sp.csr_matrix(P)
+    lens = np.diff(P.indptr)
+    unit = np.flatnonzero(lens == 1)
+    unit = unit[np.abs(P.data[P.indptr[unit]] - 1.0) <= 1e-12]
+    cols = P.indices[P.indptr[unit]]
+    first = np.full(P.shape[1], -1, dtype=np.int64)
+    order = np.argsort(cols, kind="stable")
+    c_sorted, u_sorted = cols[order], unit[order]
+    keep = np.concatenate(([True], c_sorted[1:] != c_sorted[:-1])) if c_sorted.size else np.zeros(0, bool)
+    first[c_sorted[keep]] = u_sorted[keep]
+    if np.any(first < 0):
+        raise ValueError("a coarse DOF coincides with no fine DOF; pass coarse_near_nullspace")
+    return first
 
 
 def _aggregate_sizes(n: int, agg: int) -> np.ndarray:
@@ -215,7 +232,18 @@ class SmoothedAggregationPC:
 
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
                  max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
-                 use_graph: bool = True):
+                 use_graph: bool = True, fixed_prolongators: list | None = None,
+                 coarse_near_nullspace: np.ndarray | None = None):
+        """``fixed_prolongators``: host CSR transfers (n_l x n_{l+1}) of the finest levels, used as
+        given (no aggregation, no smoothing; e.g. the same-mesh P4 -> P2 -> P1 interpolations of
+        ``pmg.py``); aggregation continues below the last of them with ``coarse_near_nullspace``
+        (rows of the coarsest fixed level; default: the near-nullspace injected through the
+        transfers' unit rows)."""
+        self.fixed = [sp.csr_matrix(P) for P in (fixed_prolongators or [])]
+        for P in self.fixed:
+            P.sort_indices()
+        self.B_coarse_fixed = None if coarse_near_nullspace is None else np.asarray(coarse_near_nullspace,
+                                                                                  dtype=np.float64)
         self.B = None if near_nullspace is None else np.asarray(near_nullspace, dtype=np.float64)
         self.use_graph = bool(use_graph)
         self._graph = None
@@ -249,16 +277,26 @@ class SmoothedAggregationPC:
         aggs = self._aggs(k)
         prec = self.precision
         levels = []
-        for lvl in range(self.max_levels):
+        nfix = len(self.fixed)
+        for lvl in range(self.max_levels + nfix):
             nl = A_pat.shape[0]
-            if nl <= self.max_coarse:
-                break
-            agg = max(aggs[min(lvl, len(aggs) - 1)], k)
-            Pt, _ = tentative_prolongator(np.ones((nl, k)), agg)
-            nc = Pt.shape[1]
-            if nc >= nl:
-                break
-            P_pat = _pattern_product(A_pat, Pt)
+            fixed = lvl < nfix
+            if fixed:
+                Pf = self.fixed[lvl]
+                if Pf.shape[0] != nl:
+                    raise ValueError(f"fixed prolongator {lvl} has {Pf.shape[0]} rows for a level of size {nl}")
+                agg, nc = 0, Pf.shape[1]
+                Pt = Pf
+                P_pat = _pattern(Pf)
+            else:
+                if nl <= self.max_coarse:
+                    break
+                agg = max(aggs[min(lvl - nfix, len(aggs) - 1)], k)
+                Pt, _ = tentative_prolongator(np.ones((nl, k)), agg)
+                nc = Pt.shape[1]
+                if nc >= nl:
+                    break
+                P_pat = _pattern_product(A_pat, Pt)
             AP_pat = _pattern_product(A_pat, P_pat)
             R_pat = _pattern(P_pat.T.tocsr())
             Ac_pat = _pattern_product(R_pat, AP_pat)
@@ -270,6 +308,8 @@ class SmoothedAggregationPC:
                 "n": nl,
                 "nc": nc,
                 "agg": agg,
+                "fixed": fixed,
+                "Pfix": torch.as_tensor(Pf.data.copy()).to(device) if fixed else None,
                 # FP64 masters for the products
                 "Pt": _DevMat(Pt.indptr, Pt.indices, nc, device, "fp64"),
                 "P64": _DevMat(P_pat.indptr, P_pat.indices, nc, device, "fp64"),
@@ -286,6 +326,9 @@ class SmoothedAggregationPC:
                 "diag": torch.as_tensor(diag).to(device),
                 "wd": torch.zeros(nl, dtype=torch.float64, device=device),
                 "Bc": torch.zeros((nc, k), dtype=torch.float64, device=device),
+                # fixed levels: the coarse near-nullspace rows taken from the fine rows that the
+                # transfer injects (a unit row: the fine DOF coincides with the coarse one)
+                "inject": torch.as_tensor(_injection_rows(Pf)).to(device) if fixed else None,
                 "v0": torch.as_tensor(np.random.default_rng(lvl).standard_normal(nl)).to(device),
             })
             A_pat = Ac_pat
@@ -307,12 +350,26 @@ class SmoothedAggregationPC:
         """P_tent values of every level (host QR) from the near-nullspace of the finest."""
         out = []
         for L in self.levels:
+            if L["fixed"]:
+                out.append(None)
+                B = self._fixed_coarse_B(L, B)
+                continue
             Pt, Bc = tentative_prolongator(B, L["agg"])
             if Pt.nnz != L["Pt"].nnz:
                 raise RuntimeError("near-nullspace changed the aggregate structure")
             out.append(Pt.data)
             B = Bc
         return out
+
+    def _fixed_coarse_B(self, L: dict, B):
+        """Near-nullspace on the level below a fixed transfer: the given one below the last
+        fixed level, else the rows of the coarse DOFs' coinciding fine DOFs."""
+        if self.B_coarse_fixed is not None and L is [M for M in self.levels if M["fixed"]][-1]:
+            Bc = self.B_coarse_fixed
+            return torch.as_tensor(Bc).to(B.device) if isinstance(B, torch.Tensor) else Bc
+        if isinstance(B, torch.Tensor):
+            return B[L["inject"]].contiguous()
+        return B[L["inject"].cpu().numpy()]
 
     @staticmethod
     def _key_of(A0, k: int) -> tuple:
@@ -355,8 +412,12 @@ class SmoothedAggregationPC:
                     self._B_host, self._B_dev = B, torch.as_tensor(np.ascontiguousarray(B)).to(device)
                 Bl = self._B_dev
             for L in self.levels:
-                native.check(native.lib().fne_aggregate_qr(L["n"], k, L["agg"], native.ptr(Bl), native.ptr(L["Pt"].data),
-                                                           native.ptr(L["Bc"]), native.stream_ptr()), "fne_aggregate_qr")
+                if L["fixed"]:
+                    L["Bc"].copy_(self._fixed_coarse_B(L, Bl))
+                else:
+                    native.check(native.lib().fne_aggregate_qr(L["n"], k, L["agg"], native.ptr(Bl),
+                                                               native.ptr(L["Pt"].data), native.ptr(L["Bc"]),
+                                                               native.stream_ptr()), "fne_aggregate_qr")
                 Bl = L["Bc"]
             pt_vals = None
         else:
@@ -374,13 +435,15 @@ class SmoothedAggregationPC:
             lam = self._lanczos_lambda_max(lvl, A0, dinv, L["v0"])
             omega = 4.0 / (3.0 * self.RHO_SAFETY * lam)
             L["wd"].copy_(omega * dinv)
-            # P = P_tent - omega D^-1 (A P_tent)
             Pt, P64 = L["Pt"], L["P64"]
-            if pt_vals is not None:
-                Pt.data.copy_(torch.as_tensor(pt_vals[lvl]))
-            _spgemm(a_ptr, a_idx, a_dat, Pt, Pt.data, P64, P64.data)
-            P64.data.mul_(-omega * dinv[L["p_rows"]])
-            P64.data.index_add_(0, L["ptpos"], Pt.data)
+            if L["fixed"]:  # a given transfer, used as it is
+                P64.data.copy_(L["Pfix"])
+            else:  # P = P_tent - omega D^-1 (A P_tent)
+                if pt_vals is not None:
+                    Pt.data.copy_(torch.as_tensor(pt_vals[lvl]))
+                _spgemm(a_ptr, a_idx, a_dat, Pt, Pt.data, P64, P64.data)
+                P64.data.mul_(-omega * dinv[L["p_rows"]])
+                P64.data.index_add_(0, L["ptpos"], Pt.data)
             # A P, R = P^T, A_c = R (A P), symmetrised
             AP64, R64, Ac64 = L["AP64"], L["R64"], L["Ac64"]
             _spgemm(a_ptr, a_idx, a_dat, P64, P64.data, AP64, AP64.data)
@@ -410,7 +473,7 @@ class SmoothedAggregationPC:
         one transfer brings the tridiagonal coefficients to the host.  A Krylov estimate: a
         few warm-started power iterations, used before, under-estimated rho once plasticity
         changed the spectrum, and the smoother (and so the V-cycle) became indefinite."""
-        s = torch.sqrt(dinv)
+        s = torch.sqrt(torch.abs(dinv))  # |D|^-1/2: finite where an indefinite Hessian has a negative diagonal
         mult = A0.mult_tensor if lvl == 0 else self.levels[lvl - 1]["Ac64"].mult
         v = v0 / torch.linalg.vector_norm(v0)
         v_prev = torch.zeros_like(v)

@@ -178,6 +178,7 @@ class ReorderedElementAssemblerBase:
         # precision of the multigrid hierarchy (None: the Hessian's); "fp32" with an FP64
         # Hessian keeps the Krylov operator exact and stores only the preconditioner in FP32
         self.ksp.pc_precision = pc_precision
+        self._configure_ksp(self.ksp)
         self._setup_timings["matrix_create"] = time.perf_counter() - t0
         # multigrid symbolic phase (patterns of the Galerkin products): mesh preprocessing,
         # done once here like the layout and the assembly plans
@@ -188,6 +189,10 @@ class ReorderedElementAssemblerBase:
     # ------------------------------------------------------------------------------------
     # problem hooks
     # ------------------------------------------------------------------------------------
+    def _configure_ksp(self, ksp) -> None:
+        """Problem-specific KSP set-up before the multigrid's symbolic phase (e.g. the level
+        transfers of pc_type "mg"); nothing by default."""
+
     def _near_nullspace(self) -> np.ndarray | None:
         """Near-nullspace vectors on the free DOFs (original order) for the AMG preconditioner:
         ``params[near_nullspace_key]`` when present (``reordered_element_base.py:1062-1090``),

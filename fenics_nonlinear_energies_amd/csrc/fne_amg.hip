@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kLanes) void spgemm_numeric_kernel(
 //   kAddProduct: y = y_in + A x                 (prolongation added to the iterate; y may be y_in)
 enum VcOp : int { kSweepFirst = 0, kSweep = 1, kResidual = 2, kAddProduct = 3 };
 
-template <typename VT, int LPR, int OP>
+template <typename VT, int LPR, int OP, bool VEC>
 __global__ __launch_bounds__(256) void vcycle_op_kernel(int n, const int32_t* __restrict__ indptr,
                                                         const int32_t* __restrict__ indices,
                                                         const VT* __restrict__ vals, const double* __restrict__ x,
@@ -119,11 +119,8 @@ __global__ __launch_bounds__(256) void vcycle_op_kernel(int n, const int32_t* __
   double acc = 0.0;
   if (row < n) {
     const int kb = indptr[row], ke = indptr[row + 1];
-    for (int k = kb + lane; k < ke; k += LPR) {
-      const int c = indices[k];
-      const double xc = OP == kSweepFirst ? w[c] * r[c] : x[c];
-      acc += double(vals[k]) * xc;
-    }
+    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, indptr[n], lane, indices, vals,
+                                    [&](int c) { return OP == kSweepFirst ? w[c] * r[c] : x[c]; });
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -151,14 +148,26 @@ void launch_vcycle_op(int n, const int32_t* indptr, const int32_t* indices, cons
     const int nb = int((threads + 255) / 256);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, n, indptr, indices, vals, x, r, w, y_in, y);
   };
-  switch (lpr) {
-    case 1: go(vcycle_op_kernel<VT, 1, OP>, 1); break;
-    case 2: go(vcycle_op_kernel<VT, 2, OP>, 2); break;
-    case 4: go(vcycle_op_kernel<VT, 4, OP>, 4); break;
-    case 8: go(vcycle_op_kernel<VT, 8, OP>, 8); break;
-    case 16: go(vcycle_op_kernel<VT, 16, OP>, 16); break;
-    case 32: go(vcycle_op_kernel<VT, 32, OP>, 32); break;
-    default: go(vcycle_op_kernel<VT, 64, OP>, 64); break;
+  if (csr_use_vec(indices, vals, lpr)) {
+    switch (lpr) {
+      case 1: go(vcycle_op_kernel<VT, 1, OP, true>, 1); break;
+      case 2: go(vcycle_op_kernel<VT, 2, OP, true>, 2); break;
+      case 4: go(vcycle_op_kernel<VT, 4, OP, true>, 4); break;
+      case 8: go(vcycle_op_kernel<VT, 8, OP, true>, 8); break;
+      case 16: go(vcycle_op_kernel<VT, 16, OP, true>, 16); break;
+      case 32: go(vcycle_op_kernel<VT, 32, OP, true>, 32); break;
+      default: go(vcycle_op_kernel<VT, 64, OP, true>, 64); break;
+    }
+  } else {
+    switch (lpr) {
+      case 1: go(vcycle_op_kernel<VT, 1, OP, false>, 1); break;
+      case 2: go(vcycle_op_kernel<VT, 2, OP, false>, 2); break;
+      case 4: go(vcycle_op_kernel<VT, 4, OP, false>, 4); break;
+      case 8: go(vcycle_op_kernel<VT, 8, OP, false>, 8); break;
+      case 16: go(vcycle_op_kernel<VT, 16, OP, false>, 16); break;
+      case 32: go(vcycle_op_kernel<VT, 32, OP, false>, 32); break;
+      default: go(vcycle_op_kernel<VT, 64, OP, false>, 64); break;
+    }
   }
 }
 

@@ -109,6 +109,86 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return start + i;
 }
 
+// ---------------------------------------------------------------------------------------------
+// CSR row dot product sum_k vals[k] * xf(indices[k]) over the row's entries [kb, ke), LPR lanes
+// per row.  VEC (long rows: the vector problems' 40-80-entry rows): lane l walks the 4-aligned
+// groups g = (kb & ~3) + 4 l + 4 LPR i with one 16-byte load of four column indices and one
+// (FP32) or two (FP64) 16-byte loads of four values; every index of such a group is a column of
+// some row (the group lies inside [0, nnz)), so the four gathers are issued unconditionally and
+// an entry outside [kb, ke) is dropped by a select afterwards -- no branch between the loads.  A
+// group running past the end of the arrays (g + 4 > nnz) is read entry by entry.  Needs indices
+// and vals 16-byte aligned (the launchers check).  !VEC (short rows, e.g. the 7-entry P1 rows):
+// one entry per lane per step.  Each lane sums its entries in increasing k; the caller reduces
+// the LPR lanes with a fixed xor tree, so the result is bitwise reproducible.
+// ---------------------------------------------------------------------------------------------
+template <typename VT> struct CsrGroup4;
+template <> struct CsrGroup4<float> {
+  static __device__ __forceinline__ void load(const float* __restrict__ p, double v[4]) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+  }
+};
+template <> struct CsrGroup4<double> {
+  static __device__ __forceinline__ void load(const double* __restrict__ p, double v[4]) {
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  }
+};
+
+template <typename VT, int LPR, bool VEC, typename XF>
+__device__ __forceinline__ double csr_row_dot(int kb, int ke, int nnz, int lane, const int32_t* __restrict__ indices,
+                                              const VT* __restrict__ vals, XF xf) {
+  double acc = 0.0;
+  if (!VEC) {
+    for (int k = kb + lane; k < ke; k += LPR) acc += double(vals[k]) * xf(indices[k]);
+    return acc;
+  }
+  for (int g = (kb & ~3) + 4 * lane; g < ke; g += 4 * LPR) {
+    if (g + 4 <= nnz) {
+      const int4 c4 = *reinterpret_cast<const int4*>(indices + g);
+      double v[4];
+      CsrGroup4<VT>::load(vals + g, v);
+      const double x0 = xf(c4.x), x1 = xf(c4.y), x2 = xf(c4.z), x3 = xf(c4.w);
+      const double t[4] = {v[0] * x0, v[1] * x1, v[2] * x2, v[3] * x3};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = g + j;
+        acc += (k >= kb && k < ke) ? t[j] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = g + j;
+        if (k >= kb && k < ke) acc += double(vals[k]) * xf(indices[k]);
+      }
+    }
+  }
+  return acc;
+}
+
+// grouped loads (VEC) for long rows, i.e. from 8 lanes per row up, when both arrays are aligned
+__host__ __forceinline__ bool csr_use_vec(const void* indices, const void* vals, int lpr) {
+  return lpr >= 8 && (reinterpret_cast<uintptr_t>(indices) & 15u) == 0 &&
+         (reinterpret_cast<uintptr_t>(vals) & 15u) == 0;
+}
+
+// lanes per CSR row: about three entries per lane for short rows; from 8 lanes up (rows of 13+
+// entries) enough groups of four for an average row and its unaligned start in one pass
+__host__ __forceinline__ int csr_lanes_per_row(double avg_row_len) {
+  int l = 1;
+  while (l < 64 && 3.0 * l < avg_row_len) l *= 2;
+  if (l >= 8)
+    while (l < 64 && 4.0 * l < avg_row_len + 3.0) l *= 2;
+  return l;
+}
+
 // gather one element value: free DOF from x, constrained DOF from the Dirichlet vector
 __device__ __forceinline__ double gather_dof(int c, int n_free, const double* __restrict__ x,
                                              const double* __restrict__ dvals) {

@@ -26,9 +26,10 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalize_pairs_kernel(const 
   if (threadIdx.x == 0) out[0] = s + c;
 }
 
-// y = A x with LPR lanes per row (LPR | 64).  Per-lane partial sums in a fixed order, then a
-// fixed xor tree inside the lane group -> bitwise reproducible.
-template <typename VT, int LPR>
+// y = A x with LPR lanes per row (LPR | 64), each lane taking groups of four entries
+// (csr_row_dot).  Per-lane partial sums in a fixed order, then a fixed xor tree inside the lane
+// group -> bitwise reproducible.
+template <typename VT, int LPR, bool VEC>
 __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int32_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ indices,
                                                       const VT* __restrict__ vals,
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int32_t* __re
   double acc = 0.0;
   if (row < n) {
     const int kb = indptr[row], ke = indptr[row + 1];
-    for (int k = kb + lane; k < ke; k += LPR) acc += double(vals[k]) * x[indices[k]];
+    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, indptr[n], lane, indices, vals, [&](int c) { return x[c]; });
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -92,14 +93,26 @@ int launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const VT* 
     const int nb = int((threads + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, stream, n, indptr, indices, vals, x, y);
   };
-  switch (lpr) {
-    case 1: go(spmv_kernel<VT, 1>, 1); break;
-    case 2: go(spmv_kernel<VT, 2>, 2); break;
-    case 4: go(spmv_kernel<VT, 4>, 4); break;
-    case 8: go(spmv_kernel<VT, 8>, 8); break;
-    case 16: go(spmv_kernel<VT, 16>, 16); break;
-    case 32: go(spmv_kernel<VT, 32>, 32); break;
-    default: go(spmv_kernel<VT, 64>, 64); break;
+  if (csr_use_vec(indices, vals, lpr)) {
+    switch (lpr) {
+      case 1: go(spmv_kernel<VT, 1, true>, 1); break;
+      case 2: go(spmv_kernel<VT, 2, true>, 2); break;
+      case 4: go(spmv_kernel<VT, 4, true>, 4); break;
+      case 8: go(spmv_kernel<VT, 8, true>, 8); break;
+      case 16: go(spmv_kernel<VT, 16, true>, 16); break;
+      case 32: go(spmv_kernel<VT, 32, true>, 32); break;
+      default: go(spmv_kernel<VT, 64, true>, 64); break;
+    }
+  } else {
+    switch (lpr) {
+      case 1: go(spmv_kernel<VT, 1, false>, 1); break;
+      case 2: go(spmv_kernel<VT, 2, false>, 2); break;
+      case 4: go(spmv_kernel<VT, 4, false>, 4); break;
+      case 8: go(spmv_kernel<VT, 8, false>, 8); break;
+      case 16: go(spmv_kernel<VT, 16, false>, 16); break;
+      case 32: go(spmv_kernel<VT, 32, false>, 32); break;
+      default: go(spmv_kernel<VT, 64, false>, 64); break;
+    }
   }
   FNE_LAUNCH_CHECK();
   return FNE_OK;
@@ -327,9 +340,8 @@ int fne_spmv(int32_t n_rows, const int32_t* indptr, const int32_t* indices, cons
              int32_t precision, const double* x, double* y, fne_stream_t stream) {
   if (n_rows < 0 || (n_rows > 0 && (!indptr || !indices || !vals || !x || !y))) return FNE_EINVAL;
   if (n_rows == 0) return FNE_OK;
-  // lanes per row from the average row length (computed on the host by the caller would need a
-  // sync; here we use a fixed policy tuned for FEM rows: 8 lanes, 16 for long rows via precision
-  // flag bit 8).
+  // lanes per row: bits 8..15 of precision (the caller knows the average row length; see
+  // csr_lanes_per_row), 8 when not given
   int lpr = (precision >> 8) & 0xFF;
   if (lpr == 0) lpr = 8;
   if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8 && lpr != 16 && lpr != 32 && lpr != 64) {

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(1024) void cg_init_finalize(CgParams P, const doubl
 
 // p_new = z + ratio p_old; w = A p_new (on the fly from z and p_old); partials of p_new.w
 // (trust mode also p_new.p_new and x.p_new, at part + gridDim.x and part + 2 gridDim.x)
-template <typename VT, int LPR>
+template <typename VT, int LPR, bool VEC>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* __restrict__ vals,
                                                          const double* __restrict__ z,
                                                          const double* __restrict__ p_old,
@@ -141,10 +141,8 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* _
   double acc = 0.0;
   if (row < P.n) {
     const int kb = P.indptr[row], ke = P.indptr[row + 1];
-    for (int k = kb + lane; k < ke; k += LPR) {
-      const int c = P.indices[k];
-      acc += double(vals[k]) * (z[c] + ratio * p_old[c]);
-    }
+    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, P.indptr[P.n], lane, P.indices, vals,
+                                    [&](int c) { return z[c] + ratio * p_old[c]; });
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
@@ -340,18 +338,17 @@ __global__ __launch_bounds__(kBlock) void cg_dots_kernel(int n, int trust, const
 
 constexpr int kVecBlocks = 1024;
 
-int lanes_for(int n, int64_t nnz) {
-  const double avg = n > 0 ? double(nnz) / n : 1.0;
-  int l = 1;
-  while (l < 64 && 3 * l < avg) l *= 2;  // about three entries per lane
-  return l;
-}
+int lanes_for(int n, int64_t nnz) { return csr_lanes_per_row(n > 0 ? double(nnz) / n : 1.0); }
 
 template <typename VT, int LPR>
 void launch_cg_spmv(const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
                     double* w, const double* x, const double* st, double2* part, int nbs, hipStream_t s) {
-  hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, x, st,
-                     part);
+  if (csr_use_vec(P.indices, vals, LPR))
+    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, true>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w,
+                       x, st, part);
+  else
+    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, false>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w,
+                       x, st, part);
 }
 
 template <typename VT>

@@ -11,8 +11,10 @@
   which tests the unpreconditioned residual norm ``||r_k||`` instead, as PETSc's KSPSTCG does
   by default.
   The preconditioner is point Jacobi (``jacobi``; zero diagonal entries are replaced by 1
-  as PETSc's PCJACOBI does), ``none``, or ``gamg`` (alias ``hypre``): the smoothed-aggregation
-  AMG V-cycle of ``amg.py`` with the problem's near-nullspace.
+  as PETSc's PCJACOBI does), ``none``, ``gamg`` (alias ``hypre``): the smoothed-aggregation
+  AMG V-cycle of ``amg.py`` with the problem's near-nullspace, ``mg``: the same V-cycle under
+  given top-level transfers (the same-mesh p-multigrid of ``pmg.py``), or ``python``: a
+  user context (``setPythonContext``).
 """
 
 from __future__ import annotations
@@ -26,6 +28,20 @@ from . import native
 from .vec import DeviceVec, dot_device
 
 
+def lanes_per_row(avg_row_len: float) -> int:
+    """Lanes per CSR row of the SpMV kernels (``csr_lanes_per_row`` of the library): about three
+    entries per lane for short rows (4 lanes for the 7-entry P1 rows); from 8 lanes up the lanes
+    read groups of four entries, and one pass covers an average row plus its unaligned start
+    (16 lanes for the 40-entry tetrahedral and 47-entry P4 vector rows)."""
+    lpr = 1
+    while lpr < 64 and 3 * lpr < avg_row_len:
+        lpr *= 2
+    if lpr >= 8:
+        while lpr < 64 and 4 * lpr < avg_row_len + 3:
+            lpr *= 2
+    return lpr
+
+
 class DeviceCSR:
     """Square CSR matrix on the GPU with a fixed pattern; values FP64 or FP32."""
 
@@ -37,12 +53,7 @@ class DeviceCSR:
         self.set_precision(precision)
         row_len = np.diff(np.asarray(indptr, dtype=np.int64))
         avg = float(row_len.mean()) if row_len.size else 1.0
-        # lanes per row: about three entries per lane (measured on MI355X: 4 lanes for the
-        # 7-entry P1 rows, 16 lanes for the 47-entry P4 vector rows)
-        lpr = 1
-        while lpr < 64 and 3 * lpr < avg:
-            lpr *= 2
-        self.lanes_per_row = lpr
+        self.lanes_per_row = lanes_per_row(avg)
 
     def set_precision(self, precision: str) -> None:
         if precision not in ("fp64", "fp32"):
@@ -96,6 +107,10 @@ class DeviceCSR:
         )
         return d
 
+    def getDiagonal(self, d: DeviceVec) -> None:
+        """PETSc ``MatGetDiagonal``: fill ``d`` with the diagonal."""
+        d.t.copy_(self.diagonal())
+
     def to_scipy(self):
         import scipy.sparse as sp
 
@@ -107,6 +122,84 @@ class DeviceCSR:
     def destroy(self) -> None:
         self.data = None
 
+
+class MatrixFreeOperator:
+    """A shell matrix: products and diagonal come from callbacks instead of stored values.
+
+    The call surface of the reference's PETSc Python matrix over ``_MatrixFreeHessianContext``
+    (``slope_stability/jax_petsc/reordered_element_assembler.py:25-50``, created at ``:313-327``):
+    ``mult`` / ``multTranspose`` (the operator is symmetric), ``getDiagonal``, ``createVecs``,
+    ``duplicate`` and ``stateIncrease``.  ``mult_fn(x, y)`` and ``diag_fn(d)`` act on device
+    FP64 tensors over the free DOFs.  A KSP given this operator runs its host-driven Krylov loops
+    (the device CG kernels read CSR values); its preconditioner comes from the optional second
+    matrix of ``KSP.setOperators`` or, for Jacobi, from ``getDiagonal``.
+    """
+
+    def __init__(self, n: int, mult_fn, diag_fn=None, *, block_size: int = 1, device="cuda", name: str = "shell"):
+        self.n = int(n)
+        self._mult = mult_fn
+        self._diag = diag_fn
+        self.block_size = int(block_size)
+        self.device = device
+        self.name = str(name)
+        self.state = 0
+        self.near_nullspace = None
+
+    def getSize(self):
+        return (self.n, self.n)
+
+    def getType(self) -> str:
+        return "python"
+
+    def getBlockSize(self) -> int:
+        return self.block_size
+
+    def setNearNullSpace(self, modes) -> None:
+        self.near_nullspace = modes
+
+    def stateIncrease(self) -> None:
+        """Mark the operator as changed (a new linearisation point)."""
+        self.state += 1
+
+    def mult_tensor(self, x: torch.Tensor, y: torch.Tensor) -> None:
+        self._mult(x, y)
+
+    def mult(self, x: DeviceVec, y: DeviceVec) -> None:
+        self._mult(x.t, y.t)
+
+    def multTranspose(self, x: DeviceVec, y: DeviceVec) -> None:
+        self._mult(x.t, y.t)
+
+    def diagonal(self) -> torch.Tensor:
+        if self._diag is None:
+            raise RuntimeError(f"matrix-free operator {self.name!r} has no diagonal")
+        d = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        self._diag(d)
+        return d
+
+    def getDiagonal(self, d: DeviceVec) -> None:
+        if self._diag is None:
+            raise RuntimeError(f"matrix-free operator {self.name!r} has no diagonal")
+        self._diag(d.t)
+
+    def createVecs(self):
+        return DeviceVec.zeros(self.n, device=self.device), DeviceVec.zeros(self.n, device=self.device)
+
+    def duplicate(self, copy: bool = False) -> "MatrixFreeOperator":
+        del copy
+        op = MatrixFreeOperator(self.n, self._mult, self._diag, block_size=self.block_size, device=self.device,
+                                name=self.name)
+        op.near_nullspace = self.near_nullspace
+        return op
+
+    def destroy(self) -> None:
+        self._mult = None
+        self._diag = None
+
+
+# preconditioners built by amg.SmoothedAggregationPC: smoothed aggregation ("gamg", alias
+# "hypre") and the p-multigrid with given top-level transfers ("mg", PETSc PCMG)
+MULTIGRID_PC_TYPES = ("gamg", "hypre", "mg")
 
 # KSP converged reasons (PETSc numbering)
 KSP_CONVERGED_RTOL = 2
@@ -133,7 +226,8 @@ class KSP:
         self.max_it = int(max_it)
         self.restart = int(restart)
         self.radius = None
-        self.A: DeviceCSR | None = None
+        self.A: DeviceCSR | MatrixFreeOperator | None = None
+        self.P: DeviceCSR | None = None  # preconditioner matrix (None: the operator itself)
         self._inv_diag: torch.Tensor | None = None
         self.its = 0
         self.reason = 0
@@ -147,6 +241,13 @@ class KSP:
         self.near_nullspace_fn = None  # optional callable -> (n, k) array at the current state, or None
         self.pc_precision = None  # multigrid storage precision ("fp64" / "fp32"); None: the operator's
         self._amg = None
+        self.python_pc = None  # pc_type "python": a context with setUp(pc) and apply(pc, b, x)
+        # pc_type "mg": host CSR transfers of the top levels (finest first; e.g. the same-mesh
+        # P4 -> P2 -> P1 interpolations), the near-nullspace below them (None: injected) and
+        # the smoothing sweeps per level; aggregation continues below the last transfer
+        self.mg_prolongators = None
+        self.mg_coarse_near_nullspace = None
+        self.mg_sweeps = 2
 
     # PETSc-style configuration -----------------------------------------------------------
     def setType(self, t: str) -> None:
@@ -162,8 +263,23 @@ class KSP:
         if max_it is not None:
             self.max_it = int(max_it)
 
-    def setOperators(self, A: DeviceCSR) -> None:
+    def setOperators(self, A, P=None) -> None:
+        """PETSc ``KSPSetOperators(A, P)``: Krylov products with ``A``, preconditioner built from
+        ``P`` (default ``A``).  ``A`` may be a ``MatrixFreeOperator``."""
         self.A = A
+        self.P = None if P is None or P is A else P
+
+    def _pmat(self):
+        return self.P if self.P is not None else self.A
+
+    def setPythonContext(self, ctx) -> None:
+        """PETSc ``PCSetPythonContext`` for pc_type "python" (``ctx.setUp(pc)``, ``ctx.apply(pc, b, x)``)."""
+        self.python_pc = ctx
+
+    def _device_loop(self) -> bool:
+        """The device CG kernels need the operator's CSR values and a preconditioner built from
+        that same matrix; anything else runs the host-driven loops."""
+        return isinstance(self.A, DeviceCSR) and self.P is None and self.pc_type != "python"
 
     def setCGRadius(self, radius: float) -> None:
         self.radius = float(radius)
@@ -172,39 +288,56 @@ class KSP:
         """Preconditioner set-up (Jacobi: inverse diagonal, zero entries -> 1)."""
         if self.A is None:
             raise RuntimeError("KSP.setUp called before setOperators")
-        self._amg = None if self.pc_type not in ("gamg", "hypre") else getattr(self, "_amg", None)
+        pmat = self._pmat()
+        self._amg = None if self.pc_type not in MULTIGRID_PC_TYPES else getattr(self, "_amg", None)
         if self.pc_type == "jacobi":
-            d = self.A.diagonal()
+            d = pmat.diagonal()
             d = torch.where(d == 0.0, torch.ones_like(d), d)
             self._inv_diag = 1.0 / d
         elif self.pc_type == "none":
             self._inv_diag = None
-        elif self.pc_type in ("gamg", "hypre"):
-            from .amg import SmoothedAggregationPC
-
-            prec = self.pc_precision or self.A.precision
+        elif self.pc_type == "python":
+            if self.python_pc is None:
+                raise RuntimeError("pc_type 'python' needs setPythonContext(ctx) first")
+            self.python_pc.setUp(None)
+            self._inv_diag = None
+        elif self.pc_type in MULTIGRID_PC_TYPES:
+            if not isinstance(pmat, DeviceCSR):
+                raise ValueError(f"pc_type {self.pc_type!r} needs an assembled preconditioner matrix "
+                                 "(setOperators(A, P))")
+            prec = self.pc_precision or pmat.precision
             if self._amg is None or self._amg.precision != prec:
                 prepared = getattr(self, "_amg_prepared", None)
                 if prepared is not None and prepared.precision == prec:
                     self._amg = prepared
                 else:
-                    self._amg = SmoothedAggregationPC(self.near_nullspace, precision=prec)
+                    self._amg = self._new_multigrid(prec)
             B = self.near_nullspace_fn() if self.near_nullspace_fn is not None else None
-            self._amg.setup(self.A, near_nullspace=B)
+            self._amg.setup(pmat, near_nullspace=B)
             self._inv_diag = None
         else:
-            raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | gamg | none)")
+            raise ValueError(f"unsupported pc_type {self.pc_type!r} (jacobi | gamg | mg | python | none)")
         self.pc_setup_count += 1
 
-    def prepare_pc(self, A: DeviceCSR) -> float:
-        """Mesh preprocessing for pc_type "gamg": the multigrid's symbolic phase (patterns of
-        every level's products) for the operator pattern of ``A``.  Returns its host time; the
-        first setUp then only runs the numeric phase."""
-        if self.pc_type not in ("gamg", "hypre"):
-            return 0.0
+    def _new_multigrid(self, precision: str):
         from .amg import SmoothedAggregationPC
 
-        pc = SmoothedAggregationPC(self.near_nullspace, precision=self.pc_precision or A.precision)
+        if self.pc_type == "mg":
+            if not self.mg_prolongators:
+                raise ValueError("pc_type 'mg' needs the level transfers (KSP.mg_prolongators)")
+            return SmoothedAggregationPC(self.near_nullspace, precision=precision,
+                                         fixed_prolongators=self.mg_prolongators,
+                                         coarse_near_nullspace=self.mg_coarse_near_nullspace,
+                                         sweeps=self.mg_sweeps)
+        return SmoothedAggregationPC(self.near_nullspace, precision=precision)
+
+    def prepare_pc(self, A: DeviceCSR) -> float:
+        """Mesh preprocessing for the multigrid preconditioners: the symbolic phase (patterns of
+        every level's products) for the operator pattern of ``A``.  Returns its host time; the
+        first setUp then only runs the numeric phase."""
+        if self.pc_type not in MULTIGRID_PC_TYPES:
+            return 0.0
+        pc = self._new_multigrid(self.pc_precision or A.precision)
         k = 1 if self.near_nullspace is None else int(np.asarray(self.near_nullspace).shape[1])
         pc.prepare(A, k)
         self._amg_prepared = pc
@@ -221,12 +354,15 @@ class KSP:
 
     def destroy(self) -> None:
         self.A = None
+        self.P = None
         self._inv_diag = None
 
     # -------------------------------------------------------------------------------------
     def _pc(self, r: torch.Tensor, z: torch.Tensor) -> None:
         if self._amg is not None:
             self._amg.apply(r, z)
+        elif self.pc_type == "python":
+            self.python_pc.apply(None, r, z)
         elif self._inv_diag is None:
             z.copy_(r)
         else:
@@ -246,7 +382,7 @@ class KSP:
 
     def solve(self, b: DeviceVec, x: DeviceVec) -> None:
         if (self._inv_diag is None and self.pc_type == "jacobi") or (
-            self._amg is None and self.pc_type in ("gamg", "hypre")
+            self._amg is None and self.pc_type in MULTIGRID_PC_TYPES
         ):
             self.setUp()
         t = self.ksp_type.lower()
@@ -254,17 +390,21 @@ class KSP:
             self._solve_cg(b.t, x.t)
         elif t == "gmres":
             self._solve_gmres(b.t, x.t)
+        elif t == "fgmres":
+            self._solve_fgmres(b.t, x.t)
         elif t == "stcg":
             self._solve_stcg(b.t, x.t)
         else:
-            raise ValueError(f"unsupported ksp_type {self.ksp_type!r} (cg | gmres | stcg)")
+            raise ValueError(f"unsupported ksp_type {self.ksp_type!r} (cg | gmres | fgmres | stcg)")
 
     # KSPCG on the device (fne_pcg): scalars stay on the GPU, the host polls the stop flag
     def _solve_cg(self, b: torch.Tensor, x: torch.Tensor) -> None:
         import ctypes as C
 
         amg = self._amg
-        if amg is not None and getattr(amg, "_graph", None) is None:  # multigrid without a recorded cycle
+        if not self._device_loop() or (amg is not None and getattr(amg, "_graph", None) is None):
+            # a shell operator, a separate preconditioner matrix or a multigrid without a
+            # recorded cycle
             self._solve_cg_host(b, x)
             return
 
@@ -433,6 +573,86 @@ class KSP:
                 self.reason = KSP_DIVERGED_ITS
                 return
 
+    # KSPFGMRES(restart): flexible GMRES, right preconditioning (the preconditioner may change
+    # between iterations, so its outputs z_k are kept and x += Z y), classical Gram-Schmidt as
+    # KSPGMRES, convergence on the unpreconditioned residual norm ||b - A x|| (PETSc's norm for
+    # right preconditioning) -- the outer solver of the reference's slope p-multigrid runs
+    # (ksp_type "fgmres", docs/problems/Plasticity.md:240-250) -------------------------------
+    def _solve_fgmres(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        A = self.A
+        m = max(1, self.restart)
+        n = b.numel()
+        x.zero_()
+        V = torch.empty((m + 1, n), dtype=torch.float64, device=b.device)
+        Z = torch.empty((m, n), dtype=torch.float64, device=b.device)
+        w = torch.empty_like(b)
+        self.its = 0
+        rnorm0 = None
+        while True:
+            if self.its == 0:
+                V[0].copy_(b)
+            else:
+                A.mult_tensor(x, V[0])
+                torch.sub(b, V[0], out=V[0])
+            beta = math.sqrt(max(self._dot(V[0], V[0]), 0.0))
+            if rnorm0 is None:
+                rnorm0 = beta
+                self.rnorm = beta
+                self.reason = self._converged(0, beta, rnorm0)
+                if self.reason:
+                    return
+            if beta == 0.0:
+                self.reason = KSP_CONVERGED_ATOL
+                return
+            V[0].mul_(1.0 / beta)
+            H = np.zeros((m + 1, m), dtype=np.float64)
+            g = np.zeros(m + 1, dtype=np.float64)
+            g[0] = beta
+            cs = np.zeros(m, dtype=np.float64)
+            sn = np.zeros(m, dtype=np.float64)
+            k_used = 0
+            for k in range(m):
+                self._pc(V[k], Z[k])
+                A.mult_tensor(Z[k], w)
+                hk = torch.mv(V[: k + 1], w)
+                w.sub_(torch.mv(V[: k + 1].T, hk))
+                hv = torch.cat((hk, torch.dot(w, w).view(1))).cpu().numpy()
+                H[: k + 1, k] = hv[: k + 1]
+                hn = math.sqrt(max(float(hv[k + 1]), 0.0))
+                H[k + 1, k] = hn
+                if hn > 0.0:
+                    torch.mul(w, 1.0 / hn, out=V[k + 1])
+                for j in range(k):
+                    t0 = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                    H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                    H[j, k] = t0
+                den = math.hypot(H[k, k], H[k + 1, k])
+                if den == 0.0:
+                    cs[k], sn[k] = 1.0, 0.0
+                else:
+                    cs[k], sn[k] = H[k, k] / den, H[k + 1, k] / den
+                H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+                H[k + 1, k] = 0.0
+                g[k + 1] = -sn[k] * g[k]
+                g[k] = cs[k] * g[k]
+                k_used = k + 1
+                self.its += 1
+                self.rnorm = abs(g[k + 1])
+                self.reason = self._converged(self.its, self.rnorm, rnorm0)
+                if self.reason or self.its >= self.max_it or hn == 0.0:
+                    break
+            y = np.zeros(k_used, dtype=np.float64)
+            for i in range(k_used - 1, -1, -1):
+                s_ = g[i] - float(np.dot(H[i, i + 1 : k_used], y[i + 1 : k_used]))
+                y[i] = s_ / H[i, i] if H[i, i] != 0.0 else 0.0
+            if k_used:
+                x.add_(torch.as_tensor(y, dtype=torch.float64, device=b.device) @ Z[:k_used])
+            if self.reason:
+                return
+            if self.its >= self.max_it:
+                self.reason = KSP_DIVERGED_ITS
+                return
+
     # Steihaug-Toint truncated CG inside radius: PETSc KSPSTCG as the trust-region Newton uses
     # it (core/petsc/trust_ksp.py:55-72), restated for the serial path in
     # core/serial/sparse_solvers.py:20-93.  Euclidean norm on the step (PETSc's default
@@ -443,7 +663,7 @@ class KSP:
         import ctypes as C
 
         amg = self._amg
-        if amg is not None and getattr(amg, "_graph", None) is None:
+        if not self._device_loop() or (amg is not None and getattr(amg, "_graph", None) is None):
             self._solve_stcg_host(b, x)
             return
         A = self.A

@@ -258,6 +258,51 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
         R[1::2, 2] = X[:, 0] - c[0]
         return R[np.asarray(self.params["freedofs"], dtype=np.int64)]
 
+    def _configure_ksp(self, ksp) -> None:
+        """pc_type "mg": the same-mesh p-multigrid levels P_k -> ... -> P1 (halving the degree,
+        4 -> 2 -> 1; ``slope_stability/jax_petsc/multigrid.py:1291-1835`` with its
+        ``same_mesh_p4_p2_p1`` hierarchy), smoothed aggregation below P1."""
+        if ksp.pc_type != "mg":
+            return
+        ksp.mg_prolongators, ksp.mg_coarse_near_nullspace = self.pmg_transfers()
+
+    def pmg_transfers(self):
+        """Host CSR transfers between the free DOFs of the same-mesh levels (finest first, each
+        coarse level in its own block Morton order) and the plane rigid-body modes on the
+        coarsest of them."""
+        from .. import pmg
+        from ..layout import perm_block_morton
+
+        p = self.params
+        degree = int(p.get("degree", 4))
+        degrees = [degree]
+        while degrees[-1] > 1:
+            degrees.append(max(1, degrees[-1] // 2))
+        geo = {k: float(p[k]) for k in ("x1", "x2", "x3", "y1", "y2")}
+        vnodes, vtris = slope_macro_mesh(float(p["h"]), **geo)
+        meshes = {degree: (np.asarray(p["nodes"]), np.asarray(p["elems_scalar"]))}
+        for k in degrees[1:]:
+            meshes[k] = lagrange_mesh(vnodes, vtris, k) if k > 1 else (vnodes, vtris)
+        order_f = pmg.free_order(p["freedofs"], self.layout.perm)
+        transfers = []
+        nodes_c = None
+        for kf, kc in zip(degrees[:-1], degrees[1:]):
+            (nf, ef), (nc, ec) = meshes[kf], meshes[kc]
+            Ps = pmg.scalar_transfer(ef, kf, ec, kc, nf.shape[0], nc.shape[0])
+            free_c = np.flatnonzero(q_mask_from_nodes(nc).ravel())
+            order_c = pmg.free_order(free_c, perm_block_morton(nc, free_c, 2))
+            transfers.append(pmg.vector_free_transfer(Ps, 2, order_f, order_c))
+            order_f, nodes_c = order_c, nc
+        if nodes_c is None:
+            return [], None
+        R = np.zeros((2 * nodes_c.shape[0], 3))
+        R[0::2, 0] = 1.0
+        R[1::2, 1] = 1.0
+        c = np.asarray(p["nodes"], dtype=np.float64).mean(axis=0)
+        R[0::2, 2] = -(nodes_c[:, 1] - c[1])
+        R[1::2, 2] = nodes_c[:, 0] - c[0]
+        return transfers, R[order_f]
+
     def _element_geometry(self) -> np.ndarray:
         K = np.asarray(self.params["K"], dtype=np.float64)
         det = np.asarray(self.params["detJ"], dtype=np.float64)
@@ -309,6 +354,171 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
             self.set_constitutive_mode(prev)
 
     # matrix-free Hessian operator (slope_stability/jax_petsc/reordered_element_assembler.py:313-562)
+    MATRIX_FREE_MODES = ("matfree_element", "matfree_overlap")
+
+    def get_matrix_free_operator(self, mode: str):
+        """The shell operator of ``mode`` (one per mode, created on first use; ``:329-333``)."""
+        from ..linalg import MatrixFreeOperator
+
+        mode = str(mode)
+        ops = self.__dict__.setdefault("_matrix_free_ops", {})
+        if mode not in ops:
+            op = MatrixFreeOperator(
+                self.layout.n_free,
+                lambda x, y, m=mode: self._matrix_free_apply(m, x, y),
+                lambda d, m=mode: self._matrix_free_diag_into(m, d),
+                block_size=self.block_size,
+                device=self.device,
+                name=mode,
+            )
+            op.setNearNullSpace(self._near_nullspace_reordered())
+            ops[mode] = op
+        return ops[mode]
+
+    def prepare_matrix_free_operator(self, u_owned, *, mode: str):
+        """Fix the linearisation point of the matrix-free Hessian and return its operator
+        (``:335-388``).  Both of the reference's modes ("matfree_element": element tangents
+        applied element by element; "matfree_overlap": the linearised gradient of the local
+        subdomain) are the same operator on one device, H(u) v summed element by element in
+        ``fne_hvp``; the mode name is kept so that operators, caches and statistics stay
+        per mode as in the reference."""
+        import time
+
+        import torch
+
+        mode = str(mode)
+        if mode not in self.MATRIX_FREE_MODES:
+            raise ValueError(f"Unsupported matrix-free operator mode {mode!r}")
+        t0 = time.perf_counter()
+        x = self._as_device(u_owned)
+        self._mf_x = torch.empty(self.layout.n_free, dtype=torch.float64, device=self.device)
+        self._mf_x.copy_(x)
+        self._mf_mode = mode
+        self._mf_state_id = getattr(self, "_mf_state_id", 0) + 1
+        self._mf_diag = None
+        self._mf_pc_cache = {}
+        self._mf_stats = {
+            "mode": mode,
+            "prepare_total": time.perf_counter() - t0,
+            "diagonal_source": "not_requested",
+            "diagonal_prepare_total": 0.0,
+            "python_pc_variant": "none",
+            "python_pc_prepare_total": 0.0,
+            "python_pc_apply_calls": 0,
+            "python_pc_apply_total": 0.0,
+            "mult_calls": 0,
+            "mult_apply": 0.0,
+            "mult_total": 0.0,
+        }
+        op = self.get_matrix_free_operator(mode)
+        op.stateIncrease()
+        return op
+
+    def matrix_free_summary(self) -> dict:
+        return dict(getattr(self, "_mf_stats", {}))
+
+    def _require_matrix_free_state(self, mode: str | None = None) -> None:
+        if getattr(self, "_mf_x", None) is None:
+            raise RuntimeError("Matrix-free state has not been prepared")
+        if mode is not None and self._mf_mode != str(mode):
+            raise RuntimeError(f"Matrix-free state {self._mf_mode!r} does not match requested mode {mode!r}")
+
+    def _matrix_free_apply(self, mode: str, x, y) -> None:
+        import time
+
+        t0 = time.perf_counter()
+        if getattr(self, "_mf_x", None) is None or self._mf_mode != str(mode):
+            raise RuntimeError(f"Matrix-free operator {mode!r} is missing a matching linearization point")
+        self.hvp_device(self._mf_x, x, y)
+        st = self._mf_stats
+        st["mult_calls"] += 1
+        dt = time.perf_counter() - t0
+        st["mult_apply"] += dt
+        st["mult_total"] += dt
+
+    def matrix_free_mult(self, mode: str, x, y) -> None:
+        """y = H(u) x at the prepared point (``:494-562``); x, y: assembler vectors."""
+        self._matrix_free_apply(mode, x.t, y.t)
+
+    def _matrix_free_diag_into(self, mode: str, d) -> None:
+        import time
+
+        import torch
+
+        self._require_matrix_free_state(mode)
+        if self._mf_diag is None or self._mf_diag[0] != self._mf_state_id:
+            t0 = time.perf_counter()
+            diag = torch.empty(self.layout.n_free, dtype=torch.float64, device=self.device)
+            self.hessian_diagonal_device(self._mf_x, diag)
+            self._mf_diag = (self._mf_state_id, diag)
+            self._mf_stats["diagonal_source"] = "element_hessian_diagonal"
+            self._mf_stats["diagonal_prepare_total"] = time.perf_counter() - t0
+        d.copy_(self._mf_diag[1])
+
+    def matrix_free_get_diagonal(self, mode: str, diag_vec) -> None:
+        """The diagonal of H(u) at the prepared point, cached per state (``:418-427``)."""
+        self._matrix_free_diag_into(mode, diag_vec.t)
+
+    def make_matrix_free_python_pc_context(self, variant: str) -> "MatrixFreePythonPC":
+        return MatrixFreePythonPC(self, variant)
+
+    def prepare_matrix_free_python_pc(self, variant: str) -> None:
+        """Factor the free-DOF Hessian at the prepared point (variant "overlap_lu", ``:451-469``).
+
+        On one device the overlap-local operator is the whole free-DOF operator.  It is
+        assembled in FP64 by ``fne_hessian`` and factored on the host by SuperLU, as the
+        reference factors its local operator with scipy's ``splu``."""
+        import time
+
+        import torch
+        from scipy.sparse.linalg import splu
+
+        from .. import native
+
+        variant = str(variant)
+        self._require_matrix_free_state()
+        cache = self._mf_pc_cache.get(variant)
+        if cache is not None and cache["state_id"] == self._mf_state_id:
+            return
+        if variant != "overlap_lu":
+            raise ValueError(f"Unsupported matrix-free python PC variant {variant!r}")
+        t0 = time.perf_counter()
+        vals = torch.empty(self.A.nnz, dtype=torch.float64, device=self.device)
+        native.check(native.lib().fne_hessian(self._plan, native.ptr(self._mf_x), native.ptr(self.dvals), None,
+                                              None, native.ptr(vals), native.FNE_FP64, native.stream_ptr()),
+                     "fne_hessian")
+        H = sparse.csr_matrix((vals.cpu().numpy(), self.A.indices.cpu().numpy(), self.A.indptr.cpu().numpy()),
+                              shape=(self.layout.n_free, self.layout.n_free))
+        self._mf_pc_cache[variant] = {"state_id": self._mf_state_id, "factor": splu(H.tocsc())}
+        self._mf_stats["python_pc_variant"] = variant
+        self._mf_stats["python_pc_prepare_total"] = time.perf_counter() - t0
+
+    def apply_matrix_free_python_pc(self, variant: str, b, x) -> None:
+        """x = LU solve of b with the prepared factor (``:471-492``)."""
+        import time
+
+        import torch
+
+        variant = str(variant)
+        self.prepare_matrix_free_python_pc(variant)
+        t0 = time.perf_counter()
+        from ..vec import DeviceVec
+
+        rhs = b.t if isinstance(b, DeviceVec) else b
+        sol = self._mf_pc_cache[variant]["factor"].solve(rhs.double().cpu().numpy())
+        (x.t if isinstance(x, DeviceVec) else x).copy_(torch.as_tensor(sol))
+        self._mf_stats["python_pc_apply_calls"] += 1
+        self._mf_stats["python_pc_apply_total"] += time.perf_counter() - t0
+
+    def cleanup(self) -> None:
+        for op in getattr(self, "_matrix_free_ops", {}).values():
+            op.destroy()
+        self.__dict__.pop("_matrix_free_ops", None)
+        self._mf_x = None
+        self._mf_pc_cache = {}
+        self._mf_stats = {}
+        super().cleanup()
+
     def hvp_device(self, x, v, y) -> None:
         """y = H(x) v without assembling H (``fne_hvp``); device FP64 tensors on the free DOFs."""
         from .. import native
@@ -322,3 +532,28 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
 
         native.check(native.lib().fne_hessian_diag(self._plan, native.ptr(x), native.ptr(self.dvals),
                                                    native.ptr(diag), native.stream_ptr()), "fne_hessian_diag")
+
+
+class MatrixFreePythonPC:
+    """Preconditioner context over the assembler's factored operator (the reference's
+    ``_MatrixFreePythonPCContext``, ``slope_stability/jax_petsc/reordered_element_assembler.py:53-72``):
+    ``setUp`` factors, ``apply`` solves, ``view`` names the variant."""
+
+    def __init__(self, assembler: SlopeStabilityReorderedElementAssembler, variant: str):
+        self.assembler = assembler
+        self.variant = str(variant)
+
+    def setUp(self, pc=None) -> None:
+        del pc
+        self.assembler.prepare_matrix_free_python_pc(self.variant)
+
+    def apply(self, pc, b, x) -> None:
+        del pc
+        self.assembler.apply_matrix_free_python_pc(self.variant, b, x)
+
+    def view(self, pc=None, viewer=None) -> str:
+        del pc
+        text = f"Matrix-free Python PC variant: {self.variant}\n"
+        if viewer is not None:
+            viewer.printfASCII(text)
+        return text

@@ -9,6 +9,16 @@ solve failures are repaired; the repair rung divides the KSP tolerance by 10 (ne
 1e-12), doubles the KSP cap and ends the line-search interval at 1.  Those rules decide which
 of the reference's runs were re-run, so they are part of its results and are kept; the
 representation (a failure table and a policy object) is this package's.
+
+One class is added to the reference's: a trust-region solve that stops because every trial
+step was rejected ("Trust-region radius exhausted before full convergence", "Trust-region
+rejected all candidate steps").  With the line search inside the radius (the reference's
+``trust_subproblem_line_search``, ``core/petsc/minimizers.py:462-481``) this happens when a
+short, nearly exact Newton step is taken at alpha = 2: the quadratic model predicts no
+reduction there (m(2) = m(0) for the exact step), the ratio test rejects every trial, and
+shrinking the radius does not change the choice while it exceeds 2 ||p||.  The repair rung's
+line-search interval ends at 1, which removes that case.  ``RepairPolicy(repair_trust=False)``
+is the reference's policy exactly.
 """
 
 from __future__ import annotations
@@ -22,6 +32,7 @@ FAILURE_PHRASES: dict[str, tuple[str, ...]] = {
     "nonfinite": ("non-finite", "nonfinite", "nan"),
     "iteration_cap": ("maximum number of iterations reached",),
     "linear_failure": ("linear solve failed",),
+    "trust_exhausted": ("trust-region radius exhausted", "trust-region rejected all candidate steps"),
 }
 
 
@@ -57,6 +68,7 @@ class RepairPolicy:
 
     repair_nonfinite: bool = True
     repair_caps: bool = True  # the Newton iteration cap and linear solve failures
+    repair_trust: bool = True  # trust-region solves ended by rejected trials (not in the reference's policy)
     rtol_factor: float = 0.1
     rtol_floor: float = 1e-12
     linesearch_upper: float = 1.0
@@ -68,6 +80,8 @@ class RepairPolicy:
             return self.repair_nonfinite
         if kind in ("iteration_cap", "linear_failure"):
             return self.repair_caps
+        if kind == "trust_exhausted":
+            return self.repair_trust
         return False
 
     def repair_rung(self, primary: Rung) -> Rung | None:
@@ -83,7 +97,7 @@ class RepairPolicy:
         return None
 
     def ladder(self, primary: Rung) -> list[Rung]:
-        if not (self.repair_nonfinite or self.repair_caps):
+        if not (self.repair_nonfinite or self.repair_caps or self.repair_trust):
             return [primary]
         rep = self.repair_rung(primary)
         return [primary] if rep is None else [primary, rep]

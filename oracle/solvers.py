@@ -6,6 +6,8 @@
 * ``gmres``: PETSc KSPGMRES(restart) with left preconditioning and its default
   orthogonalization, classical Gram-Schmidt without refinement
   (KSPGMRESClassicalGramSchmidtOrthogonalization, KSP_GMRES_CGS_REFINE_NEVER).
+* ``fgmres``: PETSc KSPFGMRES(restart), right preconditioning by any (possibly varying)
+  callable, the same orthogonalization, convergence on the unpreconditioned residual norm.
 * ``stcg``: PETSc KSPSTCG, the trust-region (Steihaug-Toint) CG of the trust-region Newton.
 * ``newton_golden``: the line-search branch of core/petsc/minimizers.py:181-1038
   (golden-section search on [a, b] after non-finite interval repair, convergence on
@@ -16,7 +18,9 @@
   after trust_max_reject + 1 rejected trials.
 * ``trust_1d`` / ``trust_2d``: the model subproblems of :129-178.
 
-Pins (tests/test_oracle_solvers.py): Krylov solvers by exact solves and termination counts;
+Pins (tests/test_oracle_solvers.py): Krylov solvers by exact solves and termination counts,
+``fgmres`` also by its residual history against brute-force least squares over the
+right-preconditioned Krylov space x in M K_k(A M, b) (restart larger than the run);
 ``newton_golden`` by a quadratic (one exact Newton step, convergence at the second
 iteration), the quartic x^4/4 whose line search is clipped at alpha = 2 (x_k = x_0 / 3^k) and
 the tolg / tolg_rel targets; ``newton_trust`` by the isotropic quadratic whose boundary steps
@@ -129,6 +133,74 @@ def gmres(A, b, rtol=1e-3, max_it=10000, restart=30, atol=1e-50, inv_diag=None):
             s = g[i] - float(H[i, i + 1 : k_used] @ y[i + 1 : k_used])
             y[i] = s / H[i, i] if H[i, i] != 0.0 else 0.0
         x += y @ V[:k_used]
+        if done:
+            return x, its, done
+        if its >= max_it:
+            return x, its, -3
+
+
+def fgmres(A, b, precond, rtol=1e-3, max_it=10000, restart=30, atol=1e-50, history=None):
+    """KSPFGMRES(restart): right preconditioning z_k = precond(v_k) (any callable, it may vary),
+    x = x0 + Z y, classical Gram-Schmidt, convergence on ||b - A x|| (the unpreconditioned norm
+    PETSc tests under right preconditioning).  ``history`` (a list) receives the Givens
+    residual estimate after each iteration.  Returns (x, its, reason)."""
+    n = b.size
+    x = np.zeros(n)
+    its = 0
+    r0 = None
+    while True:
+        r = b - A @ x if its else b.copy()
+        beta = math.sqrt(float(r @ r))
+        if r0 is None:
+            r0 = beta
+            if beta <= max(rtol * r0, atol):
+                return x, 0, 2
+        if beta == 0.0:
+            return x, its, 3
+        m = restart
+        V = np.zeros((m + 1, n))
+        Z = np.zeros((m, n))
+        V[0] = r / beta
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        g[0] = beta
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        k_used = 0
+        done = 0
+        for k in range(m):
+            Z[k] = precond(V[k])
+            w = A @ Z[k]
+            H[: k + 1, k] = V[: k + 1] @ w  # classical Gram-Schmidt
+            w = w - H[: k + 1, k] @ V[: k + 1]
+            hn = math.sqrt(float(w @ w))
+            H[k + 1, k] = hn
+            if hn > 0.0:
+                V[k + 1] = w / hn
+            for j in range(k):
+                t0 = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = t0
+            den = math.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = (1.0, 0.0) if den == 0.0 else (H[k, k] / den, H[k + 1, k] / den)
+            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            k_used = k + 1
+            its += 1
+            res = abs(g[k + 1])
+            if history is not None:
+                history.append(res)
+            if res <= max(rtol * r0, atol):
+                done = 2
+            if done or its >= max_it or hn == 0.0:
+                break
+        y = np.zeros(k_used)
+        for i in range(k_used - 1, -1, -1):
+            s = g[i] - float(H[i, i + 1 : k_used] @ y[i + 1 : k_used])
+            y[i] = s / H[i, i] if H[i, i] != 0.0 else 0.0
+        x += y @ Z[:k_used]
         if done:
             return x, its, done
         if its >= max_it:

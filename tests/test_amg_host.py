@@ -73,3 +73,83 @@ def test_coarse_inverse_cholesky_and_indefinite_fallback():
     expect = (q[:, keep] / lam2[keep]) @ q[:, keep].T
     assert fb2 and np.allclose(inv2, expect, rtol=1e-8, atol=1e-8 * np.abs(expect).max())
     assert np.allclose(inv2, inv2.T, atol=1e-10) and np.linalg.eigvalsh(inv2).min() > -1e-8
+
+
+def test_coarse_inverse_negative_trace_keeps_a_positive_cutoff():
+    """An indefinite coarse operator with a negative trace: the cutoff stays positive, so no
+    negative eigenvalue is inverted and the result is positive semi-definite."""
+    from fenics_nonlinear_energies_amd.amg import coarse_inverse
+
+    rng = np.random.default_rng(1)
+    n = 40
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.concatenate((-np.geomspace(1.0, 50.0, n - 5), np.geomspace(1e-2, 1.0, 5)))
+    A = (q * lam) @ q.T
+    A = 0.5 * (A + A.T)
+    assert np.trace(A) < 0
+    inv, fb = coarse_inverse(A)
+    keep = lam > 1e-12 * np.abs(lam).max()
+    expect = (q[:, keep] / lam[keep]) @ q[:, keep].T
+    assert fb and np.allclose(inv, expect, rtol=1e-8, atol=1e-8 * np.abs(expect).max())
+    assert np.linalg.eigvalsh(inv).min() > -1e-8
+
+
+def test_lanczos_estimate_is_finite_with_a_negative_diagonal():
+    """rho(D^-1 A) by Lanczos on |D|^-1/2 A |D|^-1/2: finite when an indefinite operator has a
+    negative diagonal entry, and equal to the exact spectral radius on a small SPD operator."""
+    import torch
+
+    class _HostOp:
+        def __init__(self, M):
+            self.M = M
+
+        def mult_tensor(self, x, y):
+            y.copy_(torch.as_tensor(self.M @ x.numpy()))
+
+    rng = np.random.default_rng(2)
+    n = 12  # Lanczos with 12 steps on 12 unknowns is exact
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    S = (q * np.linspace(0.5, 3.0, n)) @ q.T
+    pc = amg.SmoothedAggregationPC()
+    d = np.diag(S).copy()
+    lam = pc._lanczos_lambda_max(0, _HostOp(S), torch.as_tensor(1.0 / d), torch.as_tensor(rng.standard_normal(n)))
+    exact = np.max(np.abs(np.linalg.eigvals(np.diag(1.0 / d) @ S)))
+    assert abs(lam - exact) <= 1e-8 * exact
+    T = S.copy()
+    T[3, 3] = -1.0
+    lam2 = pc._lanczos_lambda_max(0, _HostOp(T), torch.as_tensor(1.0 / np.diag(T)),
+                                  torch.as_tensor(rng.standard_normal(n)))
+    assert np.isfinite(lam2) and lam2 > 0
+
+
+def test_fixed_prolongator_levels_symbolic_phase():
+    """A fixed transfer heads the hierarchy unchanged (pattern and values), aggregation goes on
+    below it, and the injection rows pick each coarse DOF's coinciding fine DOF."""
+    import torch
+
+    from fenics_nonlinear_energies_amd import pmg
+    from fenics_nonlinear_energies_amd.problems import slope_stability as S
+
+    vn, vt = S.slope_macro_mesh(2.0, **S.REFERENCE_GEOMETRY)
+    n2, e2 = S.lagrange_mesh(vn, vt, 2)
+    P21 = pmg.scalar_transfer(e2, 2, vt, 1, len(n2), len(vn))
+    inj = amg._injection_rows(P21)
+    np.testing.assert_allclose(n2[inj], vn, atol=1e-12)
+    # a scalar Laplacian-like pattern on the P2 nodes (elements' node cliques)
+    rows = np.repeat(e2, e2.shape[1], axis=1).ravel()
+    cols = np.tile(e2, (1, e2.shape[1])).ravel()
+    A = sp.csr_matrix((np.ones(rows.size), (rows, cols)), shape=(len(n2), len(n2)))
+    A.sum_duplicates()
+    A.sort_indices()
+
+    class _Pat:
+        n, nnz = A.shape[0], A.nnz
+        indptr, indices = torch.as_tensor(A.indptr.astype(np.int32)), torch.as_tensor(A.indices.astype(np.int32))
+
+    pc = amg.SmoothedAggregationPC(fixed_prolongators=[P21], max_coarse=20)
+    pc._symbolic(_Pat, "cpu", 1)
+    L0 = pc.levels[0]
+    assert L0["fixed"] and L0["n"] == len(n2) and L0["nc"] == len(vn)
+    np.testing.assert_array_equal(L0["P64"].indices.numpy(), P21.indices)
+    np.testing.assert_array_equal(L0["Pfix"].numpy(), P21.data)
+    assert len(pc.levels) >= 2 and not pc.levels[1]["fixed"] and pc.levels[1]["n"] == len(vn)

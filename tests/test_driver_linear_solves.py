@@ -165,7 +165,7 @@ def test_repair_ladder():
     rep = lad[1]
     assert rep.label == "repair" and rep.linesearch_interval == (-0.5, 1.0) and rep.ksp_max_it == 60
     assert rep.ksp_rtol == pytest.approx(1e-2)
-    assert RepairPolicy(repair_nonfinite=False, repair_caps=False).ladder(prim) == [prim]
+    assert RepairPolicy(repair_nonfinite=False, repair_caps=False, repair_trust=False).ladder(prim) == [prim]
     # rtol already at the floor, interval already inside [., 1]: only the cap moves (1 -> 2)
     tiny = RepairPolicy().ladder(primary_rung(1e-12, 1, (-0.5, 0.5)))
     assert tiny[1] == Rung("repair", 1e-12, 2, (-0.5, 0.5))
@@ -185,9 +185,14 @@ def test_which_results_trigger_a_repair():
     assert not RepairPolicy(repair_caps=False).wants_repair({"fun": 1.0, "message": "Maximum number of iterations reached"})
     assert not RepairPolicy(repair_nonfinite=False).wants_repair({"fun": float("inf"), "message": "x"})
     assert not pol.wants_repair({"fun": 1.0, "message": "Converged (energy, step, gradient)"})
-    # the trust-region stops are not failures the reference repairs
-    assert not pol.wants_repair({"fun": 1.0, "message": "Trust-region rejected all candidate steps at Newton iteration 4"})
-    assert not pol.wants_repair({"fun": 1.0, "message": "Trust-region radius exhausted before full convergence"})
+    # trust-region solves ended by rejected trials: repaired here, not by the reference's policy
+    # (RepairPolicy(repair_trust=False))
+    for msg in ("Trust-region rejected all candidate steps at Newton iteration 4",
+                "Trust-region radius exhausted before full convergence"):
+        assert classify_failure({"fun": 1.0, "message": msg}) == "trust_exhausted"
+        assert pol.wants_repair({"fun": 1.0, "message": msg})
+        assert not RepairPolicy(repair_trust=False).wants_repair({"fun": 1.0, "message": msg})
+    assert not pol.wants_repair({"fun": 1.0, "message": "Trust-region step converged"})
 
 
 class _HostVec:

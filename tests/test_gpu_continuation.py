@@ -25,8 +25,11 @@ from tests.test_oracle_mc import mc_params
 pytestmark = pytest.mark.gpu
 
 
-def test_elastic_guess_and_continuation_match_oracle(require_gpu):
-    p, adj = mc_params(h=4.0, degree=2, lam=1.0)
+# P2 below the factor of safety (about 1.2 for this slope); P4 over the strength-reduction
+# range of the P4 continuation workload (lambda 0.5 -> 1.0)
+@pytest.mark.parametrize("degree,lambdas", [(2, (1.0, 1.05, 1.1)), (4, (0.5, 0.75, 1.0))])
+def test_elastic_guess_and_continuation_match_oracle(require_gpu, degree, lambdas):
+    p, adj = mc_params(h=4.0, degree=degree, lam=1.0)
     asm = SlopeStabilityReorderedElementAssembler(p, None, adj, hessian_precision="fp64", ksp_type="cg")
     lay = asm.layout
     x, its = elastic_initial_guess(asm, rtol=1e-12, max_it=50000)
@@ -35,7 +38,7 @@ def test_elastic_guess_and_continuation_match_oracle(require_gpu):
     u_el = spla.spsolve(K.tocsc(), f)
     assert np.abs(x.array[lay.iperm] - u_el).max() <= 1e-7 * np.abs(u_el).max()
 
-    lambdas = [1.0, 1.05, 1.1]  # below the factor of safety (about 1.2 for this slope)
+    lambdas = list(lambdas)
     n_it = 8
     opts = dict(tolg=0.0, tolg_rel=0.0, tolf=0.0, tolx_rel=0.0, tolx_abs=0.0, maxit=n_it,
                 require_all_convergence=False)

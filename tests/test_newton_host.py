@@ -287,3 +287,65 @@ def test_trust_region_model_branch_equals_the_oracle_trust_newton():
     assert res["message"] == ref["message"] and res["nit"] == ref["nit"]
     assert [h["trust_radius"] for h in res["history"]] == ref["radii"]
     assert abs(res["fun"] - ref["fun"]) <= 1e-13 * abs(ref["fun"])
+
+
+def test_trust_line_search_at_alpha_two_stalls_and_the_repair_rung_does_not():
+    """The stall the trust-exhaustion repair is for, on the 1-D energy E(x) = -x + x^2/2 -
+    0.2 x^3 + 0.025 x^4 (minimum near x = 4.2) from x0 = 0, radius 200: Newton's step p = 1 is
+    exact for the quadratic model, the energy keeps falling past alpha = 1, so the golden-section
+    search on [-0.5, 2] takes alpha = 2, where the model predicts m(2) - m(0) = 0.  Every trial
+    is rejected while the radius exceeds 2 |p| (seven halvings are not enough) and the solve ends
+    with the reference's exhaustion message.  The repair rung's interval [-0.5, 1] takes
+    alpha <= 1, the ratio test passes and the solve converges."""
+    from oracle import solvers
+
+    E = lambda u: float(-u[0] + 0.5 * u[0] ** 2 - 0.2 * u[0] ** 3 + 0.025 * u[0] ** 4)
+    G = lambda u: np.array([-1.0 + u[0] - 0.6 * u[0] ** 2 + 0.1 * u[0] ** 3])
+    H = lambda u: 1.0 - 1.2 * u[0] + 0.3 * u[0] ** 2
+    Hm = lambda u, v: np.array([H(u) * v[0]])
+    ts = lambda u, rhs, r: (np.clip(rhs / H(u), -r, r), 1)
+    kw = dict(radius=200.0, line_search=True, ls_tol=1e-3, tolf=1e-12, tolg=1e-8, tolx_rel=1e-12, tolx_abs=1e-14,
+              maxit=50)
+    stall = solvers.newton_trust(E, G, Hm, np.array([0.0]), trust_solve=ts, ls=(-0.5, 2.0), **kw)
+    assert stall["message"] == "Trust-region radius exhausted before full convergence", stall["message"]
+    assert stall["nit"] == 1 and stall["x"][0] == 0.0
+    ok = solvers.newton_trust(E, G, Hm, np.array([0.0]), trust_solve=ts, ls=(-0.5, 1.0), **kw)
+    assert "converged" in ok["message"].lower(), ok["message"]
+    assert abs(G(ok["x"])[0]) < 1e-8
+
+
+def test_product_driver_stalls_the_same_way_and_the_repair_ladder_recovers():
+    """The product Newton (host vectors) on the 1-D energy above: the primary rung's interval
+    [-0.5, 2] ends in the same exhaustion message as the oracle; with the default repair policy
+    the step is solved again from its start on the repair rung ([-0.5, 1]) and converges."""
+    from fenics_nonlinear_energies_amd.repair import RepairPolicy, primary_rung, solve_with_repair
+
+    E = lambda u: float(-u[0] + 0.5 * u[0] ** 2 - 0.2 * u[0] ** 3 + 0.025 * u[0] ** 4)
+    G = lambda u: np.array([-1.0 + u[0] - 0.6 * u[0] ** 2 + 0.1 * u[0] ** 3])
+    H = lambda u: 1.0 - 1.2 * u[0] + 0.3 * u[0] ** 2
+
+    def trust_solve(x, rhs, sol, radius):
+        sol.a[:] = np.clip(rhs.a / H(x.a), -radius, radius)
+        return 1
+
+    def matvec(x, vin, vout):
+        vout.a[:] = H(x.a) * vin.a
+
+    def gradient_fn(v, g):
+        g.a[:] = G(v.a)
+
+    x = NumpyVec([0.0])
+
+    def solve_once(rung):
+        return newton(energy_fn=lambda v: E(v.a), gradient_fn=gradient_fn, hessian_solve_fn=None, x=x,
+                      hessian_matvec_fn=matvec, trust_subproblem_solve_fn=trust_solve, trust_region=True,
+                      trust_radius_init=200.0, trust_subproblem_line_search=True, linesearch_tol=1e-3,
+                      linesearch_interval=rung.linesearch_interval, tolf=1e-12, tolg=1e-8, tolx_rel=1e-12,
+                      tolx_abs=1e-14, maxit=50, require_all_convergence=True)
+
+    prim = primary_rung(1e-1, 30, (-0.5, 2.0))
+    alone = solve_once(prim)
+    assert alone["message"] == "Trust-region radius exhausted before full convergence" and x.a[0] == 0.0
+    res, used, tries = solve_with_repair(x, RepairPolicy().ladder(prim), solve_once, RepairPolicy().wants_repair)
+    assert used.label == "repair" and tries == 2 and "converged" in res["message"].lower(), res["message"]
+    assert abs(G(x.a)[0]) < 1e-8

@@ -298,3 +298,45 @@ def test_newton_trust_model_branch_takes_the_newton_step():
                                solve=lambda x, r: (np.linalg.solve(A, r), 1), radius=100.0, maxit=1, ls_tol=1e-8,
                                tolf=0.0, tolg=0.0, tolx_rel=0.0, tolx_abs=0.0)
     assert np.linalg.norm(out["x"] - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_fgmres_residuals_are_the_krylov_least_squares_minimum():
+    """Without a restart FGMRES with a fixed preconditioner M minimises ||b - A x|| over
+    x in M K_k(A M, b): its residual after k iterations equals the least-squares minimum over
+    that space, computed here by brute force (dense QR of the basis)."""
+    rng = np.random.default_rng(5)
+    n = 40
+    A = rng.standard_normal((n, n)) + n * np.eye(n) * 0.3
+    Minv = np.diag(1.0 / np.abs(np.diag(A)))
+    b = rng.standard_normal(n)
+    hist = []
+    x, its, reason = solvers.fgmres(A, b, lambda v: Minv @ v, rtol=1e-12, max_it=25, restart=60, history=hist)
+    AM = A @ Minv
+    # an orthonormal basis of K_k(A M, b) by modified Gram-Schmidt applied twice (the power
+    # basis itself is too ill-conditioned past k ~ 15)
+    Q = [b / np.linalg.norm(b)]
+    for k in range(1, len(hist) + 1):
+        Y = Minv @ np.stack(Q, axis=1)  # x in M K_k(A M, b)
+        coef, *_ = np.linalg.lstsq(A @ Y, b, rcond=None)
+        best = np.linalg.norm(b - A @ (Y @ coef))
+        assert abs(hist[k - 1] - best) <= 1e-8 * np.linalg.norm(b), (k, hist[k - 1], best)
+        v = AM @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - (q @ v) * q
+        Q.append(v / np.linalg.norm(v))
+    assert np.isclose(np.linalg.norm(b - A @ x), hist[-1], rtol=1e-6, atol=1e-12)
+
+
+def test_fgmres_with_identity_equals_gmres_and_solves():
+    """Identity preconditioner: FGMRES and (left-preconditioned, identity) GMRES are the same
+    method -- same iteration count and solution -- and a tight tolerance solves the system."""
+    rng = np.random.default_rng(6)
+    n = 30
+    A = rng.standard_normal((n, n)) + 8 * np.eye(n)
+    b = rng.standard_normal(n)
+    x1, it1, r1 = solvers.fgmres(A, b, lambda v: v, rtol=1e-10, restart=7)
+    x2, it2, r2 = solvers.gmres(A, b, rtol=1e-10, restart=7, inv_diag=np.ones(n))
+    assert (it1, r1) == (it2, r2) and r1 == 2
+    np.testing.assert_allclose(x1, x2, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(A @ x1, b, atol=1e-8 * np.linalg.norm(b))
