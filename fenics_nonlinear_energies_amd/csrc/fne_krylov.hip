@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace {
 
@@ -568,4 +569,74 @@ extern "C" int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* 
   return pcg_graph_run("fne_pstcg_graph", 1, radius, n, indptr, indices, vals, precision, pc_graph_exec, pc_in,
                        pc_out, b, x, rtol, atol, dtol, max_it, check_every, work, work_bytes, its_out, reason_out,
                        rnorm_out, stream);
+}
+
+// ---------------------------------------------------------------------------------------------
+// fne_pcg_noalloc: fne_pcg with its workspace in a file-scope device array instead of a caller
+// buffer, for callers that cannot allocate device memory.  Each device holds its own copy of
+// the array from module load to process exit; its address on the current device is looked up
+// once per device with hipGetSymbolAddress and kept in g_scratch_ptr.  Calls on one device share
+// the array, so calls that overlap in time on one device (two streams, two threads) race.  Every
+// slot the solve reads is written by the solve first (r, z, x, p, the state and the partials are
+// set by its init kernels), so nothing left by an earlier call is read.
+// ---------------------------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNoAllocScratchBytes = FNE_PCG_NOALLOC_MAX_BYTES;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    fne_set_error("fne_pcg_noalloc: no current HIP device");
+    return FNE_EHIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fne_set_error(std::string("fne_pcg_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+      return FNE_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FNE_OK;
+}
+}  // namespace
+
+extern "C" int fne_pcg_noalloc(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals,
+                               int32_t precision, const double* inv_diag, const double* b, double* x, double rtol,
+                               double atol, double dtol, int32_t max_it, int32_t check_every, int32_t* its_out,
+                               int32_t* reason_out, double* rnorm_out, fne_stream_t stream) {
+  if (n < 0 || (n > 0 && !indptr)) {
+    fne_set_error("fne_pcg_noalloc: invalid arguments");
+    return FNE_EINVAL;
+  }
+  int64_t nnz = 0;
+  if (n > 0) {
+    int32_t last = 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(&last, indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      fne_set_error("fne_pcg_noalloc: reading indptr failed");
+      return FNE_EHIP;
+    }
+    nnz = last;
+  }
+  const int64_t need = fne_pcg_workspace_bytes(n, nnz);
+  if (need < 0 || need > kNoAllocScratchBytes) {
+    fne_set_error("fne_pcg_noalloc: the solve needs " + std::to_string(need) + " bytes of scratch (n = " +
+                  std::to_string(n) + ", nnz = " + std::to_string(nnz) + "); the static scratch holds " +
+                  std::to_string(kNoAllocScratchBytes) + " bytes: use fne_pcg with a caller workspace");
+    return FNE_EINVAL;
+  }
+  void* work = nullptr;
+  const int rc = noalloc_scratch(&work);
+  if (rc != FNE_OK) return rc;
+  return pcg_run("fne_pcg_noalloc", 0, INFINITY, n, indptr, indices, vals, precision, inv_diag, b, x, rtol, atol, dtol,
+                 max_it, check_every, work, kNoAllocScratchBytes, its_out, reason_out, rnorm_out, stream);
 }

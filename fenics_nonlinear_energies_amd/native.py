@@ -28,6 +28,7 @@ FNE_FLAG_ABS_DET = 1
 FNE_FLAG_ROW_GATHER = 2
 FNE_FLAG_COLORED = 4
 FNE_FLAG_STAGED = 8
+FNE_PCG_NOALLOC_MAX_BYTES = 16 << 20
 
 _i32p = C.POINTER(C.c_int32)
 _u32p = C.POINTER(C.c_uint32)
@@ -142,6 +143,13 @@ _SIGNATURES = [
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
          C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(C.c_int32),
          C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p],
+    ),
+    (
+        "fne_pcg_noalloc",
+        C.c_int,
+        [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+         C.c_double, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+         C.POINTER(C.c_double), C.c_void_p],
     ),
     (
         "fne_pcg_graph",

@@ -291,6 +291,22 @@ int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, co
                     fne_stream_t stream);
 
 /*
+ * fne_pcg without a workspace argument, for callers that cannot allocate device memory (code
+ * that must not call hipMalloc, such as tight solve loops or harnesses that forbid it).  No
+ * counterpart in the reference.  Same arguments, semantics and outputs as fne_pcg minus
+ * work / work_bytes; the solve runs in a static device array of FNE_PCG_NOALLOC_MAX_BYTES
+ * (16 MiB per GPU, held from library load to process exit): solves needing more than
+ * fne_pcg_workspace_bytes(n, nnz) <= FNE_PCG_NOALLOC_MAX_BYTES (about 300k rows) are rejected
+ * with FNE_EINVAL.  All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (other streams or threads) race.  Allocates nothing.
+ */
+#define FNE_PCG_NOALLOC_MAX_BYTES (16ll << 20)
+int fne_pcg_noalloc(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+                    const double* inv_diag, const double* b, double* x, double rtol, double atol, double dtol,
+                    int32_t max_it, int32_t check_every, int32_t* its_out, int32_t* reason_out, double* rnorm_out,
+                    fne_stream_t stream);
+
+/*
  * Deterministic compensated FP64 dot product: out[0] = sum_i a[i]*b[i] (the PETSc VecDot /
  * VecNorm of the reference's Newton loop, core/petsc/minimizers.py:286, 329, 1003).
  * work: caller-owned device scratch of fne_dot_work_len() (sum, compensation) pairs
