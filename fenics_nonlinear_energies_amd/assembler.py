@@ -92,8 +92,9 @@ class ReorderedElementAssemblerBase:
     ):
         if comm is not None and int(comm.Get_size()) != 1:
             raise ValueError("the GPU assembler runs one rank per device; pass a one-rank communicator")
-        if local_hessian_mode != "element":
-            raise ValueError(f"Unsupported local_hessian_mode={local_hessian_mode!r} (element only)")
+        if local_hessian_mode not in ("element", "sfd_local", "sfd_local_vmap"):
+            raise ValueError(f"Unsupported local_hessian_mode={local_hessian_mode!r} "
+                             "(element | sfd_local | sfd_local_vmap)")
         if assembly_backend not in {"coo", "coo_local", "blocked_local"}:
             raise ValueError(f"Unsupported assembly backend {assembly_backend!r}")
         host_only = str(device) == "host-plan"
@@ -183,6 +184,10 @@ class ReorderedElementAssemblerBase:
         # multigrid symbolic phase (patterns of the Galerkin products): mesh preprocessing,
         # done once here like the layout and the assembly plans
         self._setup_timings["pc_symbolic"] = self.ksp.prepare_pc(self.A)
+        if self.local_hessian_mode != "element":
+            t0 = time.perf_counter()
+            self._setup_sfd()
+            self._setup_timings["sfd_coloring"] = time.perf_counter() - t0
         self._energy_out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._setup_timings["total"] = time.perf_counter() - t_total
 
@@ -490,7 +495,81 @@ class ReorderedElementAssemblerBase:
 
     def assemble_hessian(self, u_owned, variant=2):
         del variant
+        if self.local_hessian_mode != "element":
+            return self.assemble_hessian_sfd(u_owned)
         return self.assemble_hessian_element(u_owned)
+
+    # ------------------------------------------------------------------------------------
+    # Hessian-vector products and the SFD Hessian mode
+    # ------------------------------------------------------------------------------------
+    def hvp_device(self, x, v, y) -> None:
+        """y = H(x) v without assembling H (``fne_hvp``); device FP64 tensors on the free DOFs."""
+        native.check(native.lib().fne_hvp(self._plan, native.ptr(x), native.ptr(self.dvals), native.ptr(v),
+                                          native.ptr(y), native.stream_ptr()), "fne_hvp")
+
+    def _setup_sfd(self) -> None:
+        """Colouring of the SFD mode (``_setup_local_sfd``, ``reordered_element_base.py:1193-1302``):
+        a distance-2 colouring of the free-DOF pattern (``fne_color_distance2``), then per colour
+        the CSR positions whose column has that colour and their rows.  One Hessian-vector
+        product with the colour's indicator vector then gives every such entry exactly: the
+        columns of one colour never meet in a row."""
+        import ctypes as C
+
+        lay = self.layout
+        n = int(lay.n_free)
+        indptr = np.ascontiguousarray(lay.indptr, dtype=np.int32)
+        indices = np.ascontiguousarray(lay.indices, dtype=np.int32)
+        colors = np.empty(n, dtype=np.int32)
+        i32 = C.POINTER(C.c_int32)
+        k = native.lib().fne_color_distance2(n, indptr.ctypes.data_as(i32), indices.ctypes.data_as(i32),
+                                             colors.ctypes.data_as(i32))
+        native.check(min(int(k), 0), "fne_color_distance2")
+        col_color = colors[indices]
+        order = np.argsort(col_color, kind="stable")
+        bounds = np.searchsorted(col_color[order], np.arange(int(k) + 1))
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr.astype(np.int64)))
+        self._sfd_n_colors = int(k)
+        self._sfd_colors = torch.as_tensor(colors.astype(np.int64)).to(self.device)
+        self._sfd_pos = [torch.as_tensor(order[bounds[c]:bounds[c + 1]]).to(self.device) for c in range(int(k))]
+        self._sfd_rows = [torch.as_tensor(rows[order[bounds[c]:bounds[c + 1]]]).to(self.device) for c in range(int(k))]
+
+    def assemble_hessian_sfd(self, u_owned):
+        """Hessian values from one Hessian-vector product per colour (the reference's
+        ``local_hessian_mode`` "sfd_local" / "sfd_local_vmap", ``reordered_element_base.py:
+        1669-1731``, with JAX's jvp of the gradient replaced by ``fne_hvp``).  Both modes run
+        the same products here; the values equal the element assembly's up to rounding."""
+        t_total = time.perf_counter()
+        x = self._as_device(u_owned)
+        self._hessian_state = x
+        n = self.layout.n_free
+        ind = torch.empty(n, dtype=torch.float64, device=self.device)
+        y = torch.empty(n, dtype=torch.float64, device=self.device)
+        vals = torch.empty(self.A.nnz, dtype=torch.float64, device=self.device)
+        t0 = time.perf_counter()
+        for c in range(self._sfd_n_colors):
+            ind.copy_(self._sfd_colors == c)
+            self.hvp_device(x, ind, y)
+            vals[self._sfd_pos[c]] = y[self._sfd_rows[c]]
+        self.A.data.copy_(vals)
+        t_k = time.perf_counter() - t0
+        timings = {
+            "elem_hessian_compute": 0.0,
+            "scatter": 0.0,
+            "coo_assembly": 0.0,
+            "allgatherv": 0.0,
+            "ghost_exchange": 0.0,
+            "build_v_local": 0.0,
+            "p2p_exchange": 0.0,
+            "hvp_compute": t_k,
+            "extraction": 0.0,
+            "n_hvps": int(self._sfd_n_colors),
+            "assembly_mode": "sfd_overlap_local" if self.local_hessian_mode == "sfd_local"
+            else "sfd_overlap_local_vmap_hvpjit",
+            "total": time.perf_counter() - t_total,
+        }
+        self.iter_timings.append(timings)
+        self._record_callback("hessian", hvp_compute=t_k, total=timings["total"])
+        return timings
 
     def assemble_hessian_element(self, u_owned):
         t_total = time.perf_counter()

@@ -221,3 +221,6 @@ int fne_finalize_energy(fne_plan* plan, int nblocks, double* out, hipStream_t st
 size_t fne_mc_scratch_bytes(const fne_plan_desc& d, size_t* gbytes);
 // CSR rows gather their incident element rows from the scratch (n_loc 12 or 30)
 int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c);
+// Hessian-vector products of the P1 (staged plans) and tetrahedral (row-gather plans) families
+int fne_hvp_p1(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, hipStream_t s);
+int fne_hvp_tet(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, hipStream_t s);

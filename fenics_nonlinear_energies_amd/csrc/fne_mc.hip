@@ -744,7 +744,7 @@ int mc_aux(fne_plan* plan, int aux, const double* x, const double* dvals, const 
   }
   const fne_plan_desc& d = plan->d;
   if (d.kind != FNE_KIND_MOHR_COULOMB) {
-    fne_set_error("matrix-free Hessian products are implemented for the Mohr-Coulomb family");
+    fne_set_error("the matrix-free Hessian diagonal is implemented for the Mohr-Coulomb family");
     return FNE_EUNSUPPORTED;
   }
   if (!plan->mc_g || !d.inc_ptr || !d.inc) {
@@ -765,7 +765,21 @@ int mc_aux(fne_plan* plan, int aux, const double* x, const double* dvals, const 
 
 extern "C" int fne_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y,
                        fne_stream_t stream) {
-  return mc_aux(plan, 1, x, dvals, v, y, stream);
+  if (!plan || !x || !v || !y || v == y) {
+    fne_set_error("fne_hvp: invalid arguments");
+    return FNE_EINVAL;
+  }
+  const int kind = plan->d.kind;
+  if (kind == FNE_KIND_MOHR_COULOMB) return mc_aux(plan, 1, x, dvals, v, y, stream);
+  if (plan->d.n_total > 0 && !dvals) {
+    fne_set_error("Dirichlet values missing");
+    return FNE_EINVAL;
+  }
+  if (kind == FNE_KIND_PLAPLACE || kind == FNE_KIND_GINZBURG_LANDAU)
+    return fne_hvp_p1(plan, x, dvals, v, y, (hipStream_t)stream);
+  if (kind == FNE_KIND_NEOHOOKEAN) return fne_hvp_tet(plan, x, dvals, v, y, (hipStream_t)stream);
+  fne_set_error("fne_hvp: unknown energy family");
+  return FNE_EUNSUPPORTED;
 }
 
 extern "C" int fne_hessian_diag(fne_plan* plan, const double* x, const double* dvals, double* diag,

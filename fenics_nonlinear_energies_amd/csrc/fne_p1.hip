@@ -483,6 +483,53 @@ __global__ __launch_bounds__(kBlock) void p1_rowgather_kernel(fne_plan_desc d, c
   if (WANT_G) g[r] = f ? gacc - f[r] : gacc;
 }
 
+// Hessian-vector product y = H(x) v over the staged patches: phase A stages every patch
+// element's H_e v_e (element Hessian at x, v taken as zero on constrained DOFs) in LDS, phase B
+// sums each row's incidence list in its fixed order -- the gradient path of p1_staged_kernel
+// with the element gradient replaced by H_e v_e.  Deterministic, no atomics.
+template <int FAM>
+__global__ __launch_bounds__(kBlock) void p1_staged_hvp_kernel(fne_plan_desc d, const double* __restrict__ x,
+                                                               const double* __restrict__ dvals,
+                                                               const double* __restrict__ v,
+                                                               double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const int p = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int B = blockDim.x;
+  const int cap = d.chunk;
+  const int r0 = d.prow[p], r1 = d.prow[p + 1];
+  const long eb = d.pelem_ptr[p];
+  const int ne = d.pelem_ptr[p + 1] - int(eb);
+  double* sG = smem;  // [3][cap]
+  for (int sl = tid; sl < ne; sl += B) {
+    P1Elem el;
+    int code[3];
+    p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code, el);
+    double ve[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      el.v[a] = gather_dof(code[a], d.n_free, x, dvals);
+      ve[a] = code[a] < d.n_free ? v[code[a]] : 0.0;
+    }
+    double ge[3], he[6];
+    p1_grad_hess<FAM>(el, d.param, d.qdata, d.nq, false, true, ge, he);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      sG[a * cap + sl] = he[sym3(a, 0)] * ve[0] + he[sym3(a, 1)] * ve[1] + he[sym3(a, 2)] * ve[2];
+  }
+  __syncthreads();
+  const int r = r0 + tid;
+  if (r < r1) {
+    const int ib = d.inc_ptr[r], ie = d.inc_ptr[r + 1];
+    double acc = 0.0;
+    for (int k = ib; k < ie; ++k) {
+      const uint32_t it = d.inc[k];
+      acc += sG[int((it >> 13) & 3u) * cap + int(it & 0x1FFFu)];
+    }
+    y[r] = acc;
+  }
+}
+
 // dense per-element outputs (tests, autograd boundary)
 template <int FAM>
 __global__ __launch_bounds__(kBlock) void p1_element_kernel(fne_plan_desc d, const double* __restrict__ x,
@@ -685,6 +732,39 @@ int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   if (want_e) return fne_finalize_energy(plan, d.n_patch, c.out, c.stream);
   FNE_LAUNCH_CHECK();
   return FNE_OK;
+}
+
+template <int FAM>
+int launch_p1_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y,
+                  hipStream_t stream) {
+  const fne_plan_desc& d = plan->d;
+  if (!(d.flags & FNE_FLAG_STAGED) || !d.pedof || !d.pgeo || !d.inc || !d.inc_ptr) {
+    fne_set_error("P1 Hessian-vector products need a staged patch plan (FNE_FLAG_STAGED)");
+    return FNE_EUNSUPPORTED;
+  }
+  if (FAM == kFamGL && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
+    fne_set_error("Ginzburg-Landau plan needs 1..16 quadrature points in qdata");
+    return FNE_EINVAL;
+  }
+  if (d.n_patch <= 0) return FNE_OK;
+  const size_t shm = sizeof(double) * 3 * size_t(d.chunk);
+  if (shm > plan->max_lds) {
+    fne_set_error("staged patch LDS request exceeds the device limit");
+    return FNE_EINVAL;
+  }
+  hipLaunchKernelGGL(p1_staged_hvp_kernel<FAM>, dim3(d.n_patch), dim3(kBlock), shm, stream, d, x, dvals, v, y);
+  FNE_LAUNCH_CHECK();
+  return FNE_OK;
+}
+
+int fne_hvp_p1(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, hipStream_t s) {
+  if (plan->d.kind == FNE_KIND_PLAPLACE) {
+    const double p = plan->d.param[0];
+    if (p == 3.0) return launch_p1_hvp<kFamPL3>(plan, x, dvals, v, y, s);
+    if (p == 2.0) return launch_p1_hvp<kFamPL2>(plan, x, dvals, v, y, s);
+    return launch_p1_hvp<kFamPL>(plan, x, dvals, v, y, s);
+  }
+  return launch_p1_hvp<kFamGL>(plan, x, dvals, v, y, s);
 }
 
 int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {

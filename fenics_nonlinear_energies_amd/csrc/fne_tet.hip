@@ -302,6 +302,61 @@ __global__ __launch_bounds__(kBlock) void tet_store_kernel(fne_plan_desc d, cons
   }
 }
 
+// Hessian-vector product, row-gather form: one thread per element writes y_e = H_e v_e (the
+// 12x12 element Hessian at x applied to the element's v, zero on constrained DOFs) into the
+// element gradient scratch; the gradient row gather then sums each row in its fixed order.
+__global__ __launch_bounds__(kBlock) void tet_hvp_store_kernel(fne_plan_desc d, const double* __restrict__ x,
+                                                               const double* __restrict__ dvals,
+                                                               const double* __restrict__ v,
+                                                               double* __restrict__ gbuf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.n_elem) return;
+  const bool absdet = (d.flags & FNE_FLAG_ABS_DET) != 0;
+  TetElem t;
+  tet_load(d, e, x, dvals, t);
+  TetState s;
+  tet_state(t, d.param, absdet, s);
+  double ve[4][3];
+  for (int b = 0; b < 4; ++b)
+    for (int k = 0; k < 3; ++k) {
+      const int code = d.edof[size_t(3 * b + k) * d.n_elem + e];
+      ve[b][k] = code < d.n_free ? v[code] : 0.0;
+    }
+  for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < 3; ++i) {
+      double acc = 0.0;
+      for (int b = 0; b < 4; ++b) {
+        const double dd = t.D[a][0] * t.D[b][0] + t.D[a][1] * t.D[b][1] + t.D[a][2] * t.D[b][2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double h = s.s1 * s.q[b][i] * s.q[a][k] + s.s2 * s.q[a][i] * s.q[b][k];
+          if (i == k) h += s.s0 * dd;
+          acc += h * ve[b][k];
+        }
+      }
+      gbuf[size_t(e) * 32 + 3 * a + i] = acc;
+    }
+}
+
+int fne_hvp_tet(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, hipStream_t stream) {
+  const fne_plan_desc& d = plan->d;
+  if (!(d.flags & FNE_FLAG_ROW_GATHER) || !plan->mc_g || !d.inc || !d.inc_ptr) {
+    fne_set_error("tetrahedral Hessian-vector products need a row-gather plan (FNE_FLAG_ROW_GATHER)");
+    return FNE_EUNSUPPORTED;
+  }
+  const int nb = (d.n_elem + kBlock - 1) / kBlock;
+  if (nb > 0) {
+    hipLaunchKernelGGL(tet_hvp_store_kernel, dim3(nb), dim3(kBlock), 0, stream, d, x, dvals, v, plan->mc_g);
+    FNE_LAUNCH_CHECK();
+  }
+  FneCall c{};
+  c.f = nullptr;
+  c.g = y;
+  c.precision = FNE_FP64;
+  c.stream = stream;
+  return fne_launch_row_gather(plan, true, false, c);
+}
+
 int fne_launch_tet(fne_plan* plan, FneOp op, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
   if (op == kOpEnergy) {

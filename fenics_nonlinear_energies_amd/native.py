@@ -144,6 +144,7 @@ _SIGNATURES = [
          C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.POINTER(C.c_int32),
          C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p],
     ),
+    ("fne_color_distance2", C.c_int, [C.c_int32, _i32p, _i32p, _i32p]),
     (
         "fne_pcg_noalloc",
         C.c_int,

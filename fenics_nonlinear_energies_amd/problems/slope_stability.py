@@ -519,13 +519,6 @@ class SlopeStabilityReorderedElementAssembler(ReorderedElementAssemblerBase):
         self._mf_stats = {}
         super().cleanup()
 
-    def hvp_device(self, x, v, y) -> None:
-        """y = H(x) v without assembling H (``fne_hvp``); device FP64 tensors on the free DOFs."""
-        from .. import native
-
-        native.check(native.lib().fne_hvp(self._plan, native.ptr(x), native.ptr(self.dvals), native.ptr(v),
-                                          native.ptr(y), native.stream_ptr()), "fne_hvp")
-
     def hessian_diagonal_device(self, x, diag) -> None:
         """diag = diagonal of H(x) without assembling H (``fne_hessian_diag``)."""
         from .. import native

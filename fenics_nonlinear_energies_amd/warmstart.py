@@ -249,10 +249,18 @@ class NodalPredictor:
 
 
 def training_pairs(predictor: NodalPredictor, history) -> tuple[torch.Tensor, torch.Tensor]:
-    """(raw features, raw targets) of every (k-1, k, k+1) triple of a converged history."""
+    """(raw features, raw targets) of every (k-1, k, k+1) triple of consecutive, equally spaced
+    loads in a converged history.  The history holds converged steps only, so a step that did
+    not converge leaves a gap; a triple across it would pair a two-step increment with the
+    one-step extrapolation the features describe, and is skipped."""
     xs, ts = [], []
     for j in range(1, len(history) - 1):
-        (_, u_km1), (_, u_k), (load_next, u_next) = history[j - 1], history[j], history[j + 1]
+        (load_km1, u_km1), (load_k, u_k), (load_next, u_next) = history[j - 1], history[j], history[j + 1]
+        h0, h1 = float(load_k) - float(load_km1), float(load_next) - float(load_k)
+        if abs(h1 - h0) > 1e-9 * max(abs(h0), abs(h1), 1e-300):
+            continue
         xs.append(predictor.raw_features(u_k, u_km1, load_next))
         ts.append(predictor.target(u_k, u_km1, u_next))
+    if not xs:
+        raise ValueError("the history holds no triple of equally spaced loads")
     return torch.cat(xs), torch.cat(ts)

@@ -160,13 +160,16 @@ int fne_element_eval(fne_plan* plan, const double* x, const double* dvals, doubl
                      double* ge_out, double* he_out, fne_stream_t stream);
 
 /*
- * Matrix-free Hessian products, Mohr-Coulomb P2/P4 plans with FNE_FLAG_ROW_GATHER (FP64):
- * fne_hvp: y = H(x) v over the free DOFs (v is taken as zero on constrained DOFs);
- * fne_hessian_diag: diag = diagonal of H(x).  H(x) is the matrix fne_hessian assembles (the
- * return-mapping tangent at x in the plan's constitutive mode), applied element by element
- * without forming it.  Replace the matrix-free operator and its diagonal
- * (slope_stability/jax_petsc/reordered_element_assembler.py:313-562).  Other families:
- * FNE_EUNSUPPORTED.  y and v must not alias.
+ * Matrix-free Hessian products (FP64), element by element without forming H:
+ * fne_hvp: y = H(x) v over the free DOFs (v is taken as zero on constrained DOFs), for the
+ * Mohr-Coulomb P2/P4 plans (FNE_FLAG_ROW_GATHER; H is the return-mapping tangent at x in the
+ * plan's constitutive mode), the P1 families with staged patch plans (FNE_FLAG_STAGED) and the
+ * tetrahedral plans with FNE_FLAG_ROW_GATHER; other plan forms: FNE_EUNSUPPORTED.
+ * fne_hessian_diag: diag = diagonal of H(x), Mohr-Coulomb plans only.
+ * Replace the matrix-free operator and its diagonal
+ * (slope_stability/jax_petsc/reordered_element_assembler.py:313-562) and the jvp of the
+ * gradient behind the SFD Hessian mode (core/petsc/reordered_element_base.py:1295-1302).
+ * y and v must not alias.
  */
 int fne_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y, fne_stream_t stream);
 int fne_hessian_diag(fne_plan* plan, const double* x, const double* dvals, double* diag, fne_stream_t stream);
@@ -326,6 +329,20 @@ int fne_dot(int64_t n, const double* a, const double* b, double* out, void* work
  */
 int fne_color_patch_elements(int32_t n_patch, const int64_t* pelem_ptr, const int64_t* rows,
                              int32_t n_loc, int64_t n_rows, int32_t* colors);
+
+/*
+ * Host (CPU) helper of the SFD Hessian mode: greedy distance-2 colouring of a structurally
+ * symmetric CSR pattern (n rows, column indices < n).  Two rows that share a column, or whose
+ * columns meet (distance <= 2 in the pattern's graph), get different colours, so the columns
+ * of one colour are structurally orthogonal and one Hessian-vector product per colour
+ * recovers every entry.  Rows are coloured in order with the smallest colour free of their
+ * distance-2 neighbourhood.  colors[n] receives 0..k-1; returns k, or FNE_EINVAL.  Replaces
+ * the colouring of the reference's SFD set-up (igraph's greedy vertex colouring of the
+ * pattern's square, core/petsc/reordered_element_base.py:1193-1240; the C colouring
+ * core/coloring/custom_coloring.c:37-155); the colouring decides only how many products are
+ * taken, not the Hessian's values.
+ */
+int fne_color_distance2(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* colors);
 
 /*
  * Learned warm start: per-node MLP (BF16 MFMA products, FP32 activations and optimiser state).

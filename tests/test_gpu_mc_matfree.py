@@ -51,20 +51,6 @@ def test_matrix_free_hvp_and_diagonal(require_gpu, degree, mode):
     asm.cleanup()
 
 
-def test_matrix_free_rejects_other_families(require_gpu):
-    from fenics_nonlinear_energies_amd import native
-    from fenics_nonlinear_energies_amd.problems.plaplace import PLaplaceReorderedElementAssembler, build_plaplace_problem
-
-    p, adj, _ = build_plaplace_problem(8)
-    asm = PLaplaceReorderedElementAssembler(p, None, adj)
-    x = torch.zeros(asm.layout.n_free, dtype=torch.float64, device="cuda")
-    v, y = x.clone(), torch.empty_like(x)  # kept alive: distinct buffers
-    rc = native.lib().fne_hvp(asm._plan, native.ptr(x), native.ptr(asm.dvals), native.ptr(v), native.ptr(y),
-                              native.stream_ptr())
-    assert rc == native.FNE_EUNSUPPORTED
-    asm.cleanup()
-
-
 @pytest.mark.parametrize("mode", ["matfree_element", "matfree_overlap"])
 def test_prepared_operator_matches_assembled(require_gpu, mode):
     """The reference's operator API (prepare_matrix_free_operator / mult / getDiagonal,

@@ -11,6 +11,7 @@
 """
 
 import numpy as np
+import pytest
 import torch
 
 from fenics_nonlinear_energies_amd.experiments import free_node_table
@@ -99,3 +100,18 @@ def test_training_pairs_shapes_and_normalisation():
     xn, tn = pred.fit_normalization(x, t)
     assert torch.allclose(xn.mean(dim=0), torch.zeros(x.shape[1]), atol=1e-5)
     assert torch.allclose(tn.std(dim=0), torch.ones(3), atol=1e-5)
+
+
+def test_training_pairs_skip_triples_across_a_missing_step():
+    """A step that did not converge is missing from the history (loads 0, 0.5, 1.5, 2.0): the
+    triples (0, 0.5, 1.5) and (0.5, 1.5, 2.0) span the gap and are skipped; nothing usable
+    raises."""
+    pred, history, idx = _predictor("extrapolate")
+    n = idx.shape[0]
+    extra = history[-1][1] + 0.1
+    gapped = [history[0], history[1], (1.5, history[2][1]), (2.0, history[3][1]), (2.5, extra)]
+    x, t = training_pairs(pred, gapped)
+    assert x.shape[0] == n  # only (1.5, 2.0, 2.5)
+    assert torch.allclose(x[:, -1], torch.full((n,), 2.5 / 2.0))
+    with pytest.raises(ValueError):
+        training_pairs(pred, gapped[:4])
