@@ -26,9 +26,11 @@ to the host for its dense inverse.
 Level operators and transfers are kept in FP64 for the products and copied to the
 operator's precision (FP32 in the mixed-precision mode) for the cycle.
 
-Apply (device): one symmetric V-cycle -- two damped-Jacobi sweeps before and after the
-coarse correction -- with every operator product, the coarsest dense inverse included, on
-the CSR SpMV kernel (``fne_spmv``); every level, the finest included, uses the operator copied
+Apply (device): one symmetric V-cycle -- two smoothing steps before and after the coarse
+correction, Chebyshev with Jacobi scaling on [0.1, 1.1] rho(D^-1 A) by default (PETSc's
+KSPCHEBYSHEV bounds, the default PCGAMG smoother; ``fne_vcycle_cheb``) or damped Jacobi
+(``fne_vcycle_op``) -- with every operator product, the coarsest dense inverse included, on
+the CSR SpMV kernels; every level, the finest included, uses the operator copied
 at set-up, so a kept preconditioner stays the same linear operator while the Hessian changes.
 The cycle is recorded as a HIP graph after the first set-up and replayed; later set-ups
 rewrite its buffers in place.  It is a fixed symmetric positive-definite linear operator between
@@ -91,6 +93,16 @@ class _DevMat:
                                        native.ptr(r), native.ptr(w), native.ptr(y_in), native.ptr(y),
                                        native.stream_ptr()),
             "fne_vcycle_op",
+        )
+
+    def cheb(self, x, x_prev, r, w, c: float, y: torch.Tensor) -> None:
+        """Chebyshev smoothing step y = x + c (x - x_prev) + w.(r - A x) (fne_vcycle_cheb)."""
+        native.check(
+            native.lib().fne_vcycle_cheb(self.n_rows, native.ptr(self.indptr), native.ptr(self.indices),
+                                         native.ptr(self.data), self.prec | (self.lpr << 8), native.ptr(x),
+                                         native.ptr(x_prev), native.ptr(r), native.ptr(w), float(c), native.ptr(y),
+                                         native.stream_ptr()),
+            "fne_vcycle_cheb",
         )
 
     def mult(self, x: torch.Tensor, y: torch.Tensor) -> None:
@@ -233,12 +245,17 @@ class SmoothedAggregationPC:
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
                  max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
                  use_graph: bool = True, fixed_prolongators: list | None = None,
-                 coarse_near_nullspace: np.ndarray | None = None):
+                 coarse_near_nullspace: np.ndarray | None = None, smoother: str = "chebyshev"):
         """``fixed_prolongators``: host CSR transfers (n_l x n_{l+1}) of the finest levels, used as
         given (no aggregation, no smoothing; e.g. the same-mesh P4 -> P2 -> P1 interpolations of
         ``pmg.py``); aggregation continues below the last of them with ``coarse_near_nullspace``
         (rows of the coarsest fixed level; default: the near-nullspace injected through the
         transfers' unit rows)."""
+        if smoother not in ("jacobi", "chebyshev"):
+            raise ValueError(f"unsupported multigrid smoother {smoother!r} (jacobi | chebyshev)")
+        # level smoother: damped Jacobi (omega = 4 / (3 rho)), or Chebyshev on [0.1, 1.1] rho with
+        # Jacobi scaling (PETSc's KSPCHEBYSHEV eigenvalue bounds, the default PCGAMG smoother)
+        self.smoother = smoother
         self.fixed = [sp.csr_matrix(P) for P in (fixed_prolongators or [])]
         for P in self.fixed:
             P.sort_indices()
@@ -325,6 +342,7 @@ class SmoothedAggregationPC:
                 "tperm_ac": torch.as_tensor(_transpose_perm(Ac_pat)).to(device),
                 "diag": torch.as_tensor(diag).to(device),
                 "wd": torch.zeros(nl, dtype=torch.float64, device=device),
+                "wcheb": torch.zeros((max(1, self.sweeps), nl), dtype=torch.float64, device=device),
                 "Bc": torch.zeros((nc, k), dtype=torch.float64, device=device),
                 # fixed levels: the coarse near-nullspace rows taken from the fine rows that the
                 # transfer injects (a unit row: the fine DOF coincides with the coarse one)
@@ -435,6 +453,9 @@ class SmoothedAggregationPC:
             lam = self._lanczos_lambda_max(lvl, A0, dinv, L["v0"])
             omega = 4.0 / (3.0 * self.RHO_SAFETY * lam)
             L["wd"].copy_(omega * dinv)
+            if self.smoother == "chebyshev":
+                for k, coef in enumerate(self._cheb_weights(lam)):
+                    L["wcheb"][k].copy_(coef * dinv)
             Pt, P64 = L["Pt"], L["P64"]
             if L["fixed"]:  # a given transfer, used as it is
                 P64.data.copy_(L["Pfix"])
@@ -466,6 +487,30 @@ class SmoothedAggregationPC:
         self.setup_time += time.perf_counter() - t_setup
         if self.use_graph and self._graph is None and torch.device(device).type == "cuda":
             self._capture(n, device)
+
+    CHEB_BOUNDS = (0.1, 1.1)  # [a, b] = these times the estimate of rho(D^-1 A)
+
+    def _cheb_rhos(self) -> list:
+        """rho_k of the Chebyshev recurrence; they depend only on b / a, so the coefficients c_k
+        of the recorded cycle stay valid across set-ups."""
+        lo, hi = self.CHEB_BOUNDS
+        sigma = (hi + lo) / (hi - lo)
+        rhos = [1.0 / sigma]
+        for _ in range(1, max(1, self.sweeps)):
+            rhos.append(1.0 / (2.0 * sigma - rhos[-1]))
+        return rhos
+
+    def _cheb_coefs(self) -> list:
+        """c_k of step k (c_0 = 0): the weight of x_k - x_{k-1}."""
+        r = self._cheb_rhos()
+        return [0.0] + [r[k] * r[k - 1] for k in range(1, len(r))]
+
+    def _cheb_weights(self, lam: float) -> list:
+        """Scalar factors of D^-1 in step k: 1 / theta, then 2 rho_k / delta."""
+        lo, hi = self.CHEB_BOUNDS
+        theta, delta = 0.5 * (hi + lo) * lam, 0.5 * (hi - lo) * lam
+        r = self._cheb_rhos()
+        return [1.0 / theta] + [2.0 * r[k] / delta for k in range(1, len(r))]
 
     def _lanczos_lambda_max(self, lvl: int, A0, dinv: torch.Tensor, v0: torch.Tensor) -> float:
         """Largest eigenvalue of D^-1 A (similar to S = D^-1/2 A D^-1/2) from LANCZOS_STEPS
@@ -522,15 +567,35 @@ class SmoothedAggregationPC:
         self.coarse_op.mult(r, y)
         return y
 
+    def _cheb_smooth(self, L: dict, r: torch.Tensor, x: torch.Tensor | None) -> torch.Tensor:
+        """``sweeps`` Chebyshev steps on A x = r from x (from zero when x is None): step k is
+        x_{k+1} = x_k + c_k (x_k - x_{k-1}) + w_k (r - A x_k), one CSR pass (fne_vcycle_cheb);
+        from zero the first step is x_1 = w_0 r."""
+        A, W, cs = L["A"], L["wcheb"], self._cheb_coefs()
+        x_prev = None
+        start = 0
+        if x is None:
+            x = W[0] * r
+            x_prev = torch.zeros_like(r)
+            start = 1
+        for k in range(start, W.shape[0]):
+            y = x_prev if (x_prev is not None and k > 1) else torch.empty_like(r)
+            A.cheb(x, x_prev if cs[k] != 0.0 else None, r, W[k], cs[k], y)
+            x_prev, x = x, y
+        return x
+
     def _cycle(self, lvl: int, r: torch.Tensor) -> torch.Tensor:
-        """One V-cycle from level ``lvl``: a fused CSR pass per sweep, residual and prolongation
-        (x_{k+1} = x_k + w (r - A x_k), w = omega D^-1, starting from x = 0)."""
+        """One V-cycle from level ``lvl``: a fused CSR pass per smoothing step, residual and
+        prolongation.  Jacobi: x_{k+1} = x_k + w (r - A x_k), w = omega D^-1, from x = 0;
+        Chebyshev: ``_cheb_smooth``."""
         if lvl == len(self.levels):
             return self._coarse_solve(r)
         L = self.levels[lvl]
         A, wd = L["A"], L["wd"]
         t = torch.empty_like(r)
-        if self.sweeps == 1:
+        if self.smoother == "chebyshev":
+            x = self._cheb_smooth(L, r, None)
+        elif self.sweeps == 1:
             x = wd * r
         else:
             x = torch.empty_like(r)
@@ -543,6 +608,8 @@ class SmoothedAggregationPC:
         L["R"].mult(t, rc)
         xc = self._cycle(lvl + 1, rc)
         L["P"].op(_ADD_PRODUCT, xc, None, None, x, x)
+        if self.smoother == "chebyshev":
+            return self._cheb_smooth(L, r, x)
         for _ in range(self.sweeps):
             A.op(_SWEEP, x, r, wd, None, t)
             x, t = t, x

@@ -179,6 +179,7 @@ class ReorderedElementAssemblerBase:
         # precision of the multigrid hierarchy (None: the Hessian's); "fp32" with an FP64
         # Hessian keeps the Krylov operator exact and stores only the preconditioner in FP32
         self.ksp.pc_precision = pc_precision
+        self._apply_pc_options(pc_options)
         self._configure_ksp(self.ksp)
         self._setup_timings["matrix_create"] = time.perf_counter() - t0
         # multigrid symbolic phase (patterns of the Galerkin products): mesh preprocessing,
@@ -194,6 +195,20 @@ class ReorderedElementAssemblerBase:
     # ------------------------------------------------------------------------------------
     # problem hooks
     # ------------------------------------------------------------------------------------
+    def _apply_pc_options(self, pc_options) -> None:
+        """The multigrid level-smoother options among ``pc_options`` (PETSc option names, as the
+        reference passes them): ``mg_levels_ksp_type`` ("chebyshev", the default as in PCGAMG,
+        or "richardson" = damped Jacobi sweeps) and ``mg_levels_ksp_max_it`` (steps per level and
+        side, 2 by default).  Other options have no counterpart on this solver stack and are
+        ignored."""
+        opts = {str(k).lstrip("-"): v for k, v in dict(pc_options or {}).items()}
+        kind = str(opts.get("mg_levels_ksp_type", "chebyshev")).lower()
+        if kind not in ("richardson", "jacobi", "chebyshev"):
+            raise ValueError(f"unsupported mg_levels_ksp_type {kind!r} (richardson | chebyshev)")
+        self.ksp.mg_smoother = "chebyshev" if kind == "chebyshev" else "jacobi"
+        if "mg_levels_ksp_max_it" in opts:
+            self.ksp.mg_sweeps = int(opts["mg_levels_ksp_max_it"])
+
     def _configure_ksp(self, ksp) -> None:
         """Problem-specific KSP set-up before the multigrid's symbolic phase (e.g. the level
         transfers of pc_type "mg"); nothing by default."""

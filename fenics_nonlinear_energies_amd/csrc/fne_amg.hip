@@ -352,6 +352,99 @@ extern "C" int fne_vcycle_op(int32_t op, int32_t n_rows, const int32_t* indptr, 
   return FNE_OK;
 }
 
+// Chebyshev smoothing step of the multigrid levels: y = x + c (x - x_prev) + w.(r - A x), one CSR
+// pass.  With w = (2 rho_k / delta) D^-1 and c = rho_k rho_{k-1} it is step k of the Chebyshev
+// iteration for D^-1 A on [a, b] (theta = (a + b) / 2, delta = (b - a) / 2); x_prev may be null
+// when c = 0.  y may alias x_prev (each row reads its own x_prev entry before writing it).
+namespace {
+template <typename VT, int LPR, bool VEC>
+__global__ __launch_bounds__(256) void vcycle_cheb_kernel(int n, const int32_t* __restrict__ indptr,
+                                                          const int32_t* __restrict__ indices,
+                                                          const VT* __restrict__ vals, const double* __restrict__ x,
+                                                          const double* x_prev, const double* __restrict__ r,
+                                                          const double* __restrict__ w, double c, double* y) {
+  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int row = gtid / LPR;
+  const int lane = gtid % LPR;
+  double acc = 0.0;
+  if (row < n) {
+    const int kb = indptr[row], ke = indptr[row + 1];
+    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, indptr[n], lane, indices, vals, [&](int col) { return x[col]; });
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  if (row < n && lane == 0) {
+    const double xr = x[row];
+    double out = xr + w[row] * (r[row] - acc);
+    if (x_prev) out += c * (xr - x_prev[row]);
+    y[row] = out;
+  }
+}
+
+template <typename VT>
+int launch_vcycle_cheb(int n, const int32_t* indptr, const int32_t* indices, const VT* vals, const double* x,
+                       const double* x_prev, const double* r, const double* w, double c, double* y, int lpr,
+                       hipStream_t s) {
+  auto go = [&](auto kern, int L) {
+    const long long threads = (long long)n * L;
+    const int nb = int((threads + 255) / 256);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, n, indptr, indices, vals, x, x_prev, r, w, c, y);
+  };
+  const bool vec = csr_use_vec(indices, vals, lpr);
+#define CHEB_L(L)                                                   \
+  case L:                                                           \
+    if (vec) go(vcycle_cheb_kernel<VT, L, true>, L);                 \
+    else go(vcycle_cheb_kernel<VT, L, false>, L);                    \
+    break;
+  switch (lpr) {
+    CHEB_L(1)
+    CHEB_L(2)
+    CHEB_L(4)
+    CHEB_L(8)
+    CHEB_L(16)
+    CHEB_L(32)
+    default:
+      if (vec) go(vcycle_cheb_kernel<VT, 64, true>, 64);
+      else go(vcycle_cheb_kernel<VT, 64, false>, 64);
+  }
+#undef CHEB_L
+  FNE_LAUNCH_CHECK();
+  return FNE_OK;
+}
+}  // namespace
+
+extern "C" int fne_vcycle_cheb(int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
+                               int32_t precision, const double* x, const double* x_prev, const double* r,
+                               const double* w, double c, double* y, fne_stream_t stream) {
+  if (n_rows < 0) {
+    fne_set_error("fne_vcycle_cheb: n_rows < 0");
+    return FNE_EINVAL;
+  }
+  if (n_rows == 0) return FNE_OK;
+  if (!indptr || !indices || !vals || !x || !r || !w || !y || (c != 0.0 && !x_prev)) {
+    fne_set_error("fne_vcycle_cheb: null pointer");
+    return FNE_EINVAL;
+  }
+  if (y == x || y == r || y == w) {
+    fne_set_error("fne_vcycle_cheb: the output aliases an input read across rows");
+    return FNE_EINVAL;
+  }
+  int lpr = (precision >> 8) & 0xff;
+  if (lpr == 0) lpr = 8;
+  if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8 && lpr != 16 && lpr != 32 && lpr != 64) {
+    fne_set_error("fne_vcycle_cheb: lanes per row must be a power of two <= 64");
+    return FNE_EINVAL;
+  }
+  const int prec = precision & 0xff;
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == FNE_FP64)
+    return launch_vcycle_cheb<double>(n_rows, indptr, indices, (const double*)vals, x, x_prev, r, w, c, y, lpr, s);
+  if (prec == FNE_FP32)
+    return launch_vcycle_cheb<float>(n_rows, indptr, indices, (const float*)vals, x, x_prev, r, w, c, y, lpr, s);
+  fne_set_error("fne_vcycle_cheb: precision must be FNE_FP64 or FNE_FP32");
+  return FNE_EINVAL;
+}
+
 extern "C" int fne_spgemm_numeric(int32_t n_rows, const int32_t* a_indptr, const int32_t* a_indices,
                                   const void* a_vals, int32_t a_precision, const int32_t* b_indptr,
                                   const int32_t* b_indices, const double* b_vals, const int32_t* c_indptr,

@@ -247,7 +247,8 @@ class KSP:
         # the smoothing sweeps per level; aggregation continues below the last transfer
         self.mg_prolongators = None
         self.mg_coarse_near_nullspace = None
-        self.mg_sweeps = 2
+        self.mg_sweeps = 2  # smoothing steps per level, each side (-mg_levels_ksp_max_it)
+        self.mg_smoother = "chebyshev"  # "chebyshev" | "jacobi" (-mg_levels_ksp_type chebyshev | richardson)
 
     # PETSc-style configuration -----------------------------------------------------------
     def setType(self, t: str) -> None:
@@ -328,8 +329,9 @@ class KSP:
             return SmoothedAggregationPC(self.near_nullspace, precision=precision,
                                          fixed_prolongators=self.mg_prolongators,
                                          coarse_near_nullspace=self.mg_coarse_near_nullspace,
-                                         sweeps=self.mg_sweeps)
-        return SmoothedAggregationPC(self.near_nullspace, precision=precision)
+                                         sweeps=self.mg_sweeps, smoother=self.mg_smoother)
+        return SmoothedAggregationPC(self.near_nullspace, precision=precision, sweeps=self.mg_sweeps,
+                                     smoother=self.mg_smoother)
 
     def prepare_pc(self, A: DeviceCSR) -> float:
         """Mesh preprocessing for the multigrid preconditioners: the symbolic phase (patterns of

@@ -146,6 +146,12 @@ _SIGNATURES = [
     ),
     ("fne_color_distance2", C.c_int, [C.c_int32, _i32p, _i32p, _i32p]),
     (
+        "fne_vcycle_cheb",
+        C.c_int,
+        [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.c_double, C.c_void_p, C.c_void_p],
+    ),
+    (
         "fne_pcg_noalloc",
         C.c_int,
         [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,

@@ -236,6 +236,19 @@ int fne_vcycle_op(int32_t op, int32_t n_rows, const int32_t* indptr, const int32
                   int32_t precision, const double* x, const double* r, const double* w, const double* y_in,
                   double* y, fne_stream_t stream);
 
+/*
+ * Chebyshev smoothing step of the multigrid levels, one CSR pass:
+ *   y = x + c (x - x_prev) + w.(r - A x)      (x_prev may be NULL when c = 0)
+ * With w = (2 rho_k / delta) D^-1 and c = rho_k rho_{k-1} it is step k of the Chebyshev iteration
+ * for D^-1 A on [a, b]; step 0 is w = D^-1 / theta, c = 0 (theta, delta: centre and half-width of
+ * [a, b]).  Stands in for PETSc's KSPCHEBYSHEV level smoother with Jacobi, the default smoother
+ * of PCGAMG (pc_type "gamg", hyperelasticity/jax_petsc/solver.py:138-143).  y may alias x_prev,
+ * not x, r or w.  precision as in fne_vcycle_op.
+ */
+int fne_vcycle_cheb(int32_t n_rows, const int32_t* indptr, const int32_t* indices, const void* vals,
+                    int32_t precision, const double* x, const double* x_prev, const double* r, const double* w,
+                    double c, double* y, fne_stream_t stream);
+
 int fne_spgemm_numeric(int32_t n_rows, const int32_t* a_indptr, const int32_t* a_indices, const void* a_vals,
                        int32_t a_precision, const int32_t* b_indptr, const int32_t* b_indices,
                        const double* b_vals, const int32_t* c_indptr, const int32_t* c_indices, double* c_vals,

@@ -153,3 +153,32 @@ def test_fixed_prolongator_levels_symbolic_phase():
     np.testing.assert_array_equal(L0["P64"].indices.numpy(), P21.indices)
     np.testing.assert_array_equal(L0["Pfix"].numpy(), P21.data)
     assert len(pc.levels) >= 2 and not pc.levels[1]["fixed"] and pc.levels[1]["n"] == len(vn)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 4])
+def test_chebyshev_coefficients_give_the_chebyshev_error_polynomial(steps):
+    """The smoother's recurrence x_{k+1} = x_k + c_k (x_k - x_{k-1}) + w_k D^-1 (r - A x_k)
+    (x_0 = 0) with the multigrid's coefficients on a diagonal operator with D = I: the error of
+    each eigencomponent lambda in [a, b] = [0.1, 1.1] rho shrinks by
+    T_m((theta - lambda) / delta) / T_m(theta / delta), the Chebyshev closed form."""
+    pc = amg.SmoothedAggregationPC(sweeps=steps, smoother="chebyshev")
+    rho = 2.5
+    lam = np.linspace(0.1 * rho, 1.1 * rho, 11)
+    w = pc._cheb_weights(rho)
+    c = pc._cheb_coefs()
+    r = np.ones_like(lam)  # A x = r, exact x = r / lam
+    x_prev, x = np.zeros_like(lam), np.zeros_like(lam)
+    for k in range(steps):
+        y = x + c[k] * (x - x_prev) + w[k] * (r - lam * x)
+        x_prev, x = x, y
+    err = (r / lam - x) / (r / lam)
+    theta, delta = 0.6 * rho, 0.5 * rho
+    def T(m, t):  # Chebyshev polynomial of the first kind, any real argument
+        if abs(t) <= 1.0:
+            return np.cos(m * np.arccos(t))
+        return np.sign(t) ** m * np.cosh(m * np.arccosh(abs(t)))
+
+    expect = np.array([T(steps, (theta - l) / delta) for l in lam]) / T(steps, theta / delta)
+    np.testing.assert_allclose(err, expect, atol=1e-12)
+    with pytest.raises(ValueError):
+        amg.SmoothedAggregationPC(smoother="sor")

@@ -134,11 +134,13 @@ def _host_levels(pc, A0):
     return out
 
 
-def test_device_galerkin_setup_matches_host_products(require_gpu):
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+def test_device_galerkin_setup_matches_host_products(require_gpu, smoother):
     """Every level of the device set-up: P = P_tent - omega D^-1 A P_tent with P_tent from the
     host QR of the (current-configuration) rigid modes, A_c = P^T A P, omega within 15 % of the
     host power-iteration value of 4 / (3 rho(D^-1 A)) (60 steps); and the V-cycle equals a host cycle
-    on the same operators."""
+    on the same operators, with either smoother (Chebyshev in the textbook form of Saad,
+    Iterative Methods, Alg. 12.1, on [0.1, 1.1] lambda_max with Jacobi scaling)."""
     import torch
 
     from fenics_nonlinear_energies_amd import amg
@@ -150,7 +152,7 @@ def test_device_galerkin_setup_matches_host_products(require_gpu):
     asm.assemble_hessian(x)
     B = asm._rigid_modes_current().cpu().numpy()
     np.testing.assert_array_equal(B, rigid_modes(x.t.cpu().numpy().reshape(-1, 3)))
-    pc = amg.SmoothedAggregationPC(B, precision="fp64", max_coarse=100)
+    pc = amg.SmoothedAggregationPC(B, precision="fp64", max_coarse=100, smoother=smoother)
     pc.setup(asm.A)
     assert len(pc.levels) >= 2
     Bl = B
@@ -181,14 +183,37 @@ def test_device_galerkin_setup_matches_host_products(require_gpu):
     levels = _host_levels(pc, asm.A.to_scipy())
     Ainv = pc.coarse_op.data.cpu().numpy().reshape(pc._coarse_n, pc._coarse_n)
 
+    def cheb(lvl, A, r, x):
+        # Saad Alg. 12.1 with M = D: d_0 = M^-1 r_0 / theta, d_k = rho_k rho_{k-1} d_{k-1}
+        # + (2 rho_k / delta) M^-1 r_k, x_{k+1} = x_k + d_k
+        dinv = 1.0 / A.diagonal()
+        lam = 1.0 / (0.6 * float(pc.levels[lvl]["wcheb"][0, 0].cpu()) / dinv[0])
+        a, b = 0.1 * lam, 1.1 * lam
+        theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+        sigma = theta / delta
+        rho = 1.0 / sigma
+        dk = dinv * (r - A @ x) / theta
+        x = x + dk
+        for _ in range(pc.sweeps - 1):
+            rho_new = 1.0 / (2.0 * sigma - rho)
+            dk = rho_new * rho * dk + (2.0 * rho_new / delta) * dinv * (r - A @ x)
+            x = x + dk
+            rho = rho_new
+        return x
+
     def cycle(lvl, r):
         if lvl == len(levels):
             return Ainv @ r
         A, P, _, wd = levels[lvl]
-        xx = wd * r
-        for _ in range(pc.sweeps - 1):
-            xx = xx + wd * (r - A @ xx)
+        if smoother == "chebyshev":
+            xx = cheb(lvl, A, r, np.zeros_like(r))
+        else:
+            xx = wd * r
+            for _ in range(pc.sweeps - 1):
+                xx = xx + wd * (r - A @ xx)
         xx = xx + P @ cycle(lvl + 1, P.T @ (r - A @ xx))
+        if smoother == "chebyshev":
+            return cheb(lvl, A, r, xx)
         for _ in range(pc.sweeps):
             xx = xx + wd * (r - A @ xx)
         return xx
@@ -318,3 +343,85 @@ def test_fp32_multigrid_with_fp64_operator(require_gpu):
     ref = spla.spsolve(asm.A.to_scipy().tocsc(), b.array)
     assert np.linalg.norm(xg.array - ref) <= 1e-6 * np.linalg.norm(ref)
     asm.cleanup()
+
+
+@pytest.mark.parametrize("prec,density", [("fp64", 0.03), ("fp32", 0.03), ("fp64", 0.12)])
+def test_chebyshev_step_matches_numpy(require_gpu, prec, density):
+    """fne_vcycle_cheb: y = x + c (x - x_prev) + w.(r - A x) against numpy (the 0.12 density gives
+    60-entry rows, the grouped-load path), in place over x_prev, and with x_prev absent for
+    c = 0; an output aliasing x is rejected."""
+    import scipy.sparse as sp
+    import torch
+
+    from fenics_nonlinear_energies_amd import amg
+    from fenics_nonlinear_energies_amd.native import FneError
+
+    rng = np.random.default_rng(11)
+    A = sp.random(500, 500, density=density, random_state=12, format="csr") + sp.eye(500, format="csr")
+    A.sort_indices()
+    if prec == "fp32":
+        A.data = A.data.astype(np.float32).astype(np.float64)
+    dev = torch.device("cuda")
+    Ad = amg._DevMat(A.indptr, A.indices, 500, dev, prec)
+    Ad.data.copy_(torch.as_tensor(A.data))
+    x, xp, r, w = (rng.standard_normal(500) for _ in range(4))
+    T = lambda v: torch.as_tensor(v).to(dev)  # noqa: E731
+    tol = dict(rtol=1e-12, atol=1e-12)
+    y = torch.empty(500, dtype=torch.float64, device=dev)
+    Ad.cheb(T(x), T(xp), T(r), T(w), 0.7, y)
+    np.testing.assert_allclose(y.cpu().numpy(), x + 0.7 * (x - xp) + w * (r - A @ x), **tol)
+    Ad.cheb(T(x), None, T(r), T(w), 0.0, y)
+    np.testing.assert_allclose(y.cpu().numpy(), x + w * (r - A @ x), **tol)
+    yp = T(xp)
+    Ad.cheb(T(x), yp, T(r), T(w), 0.3, yp)  # in place over x_prev
+    np.testing.assert_allclose(yp.cpu().numpy(), x + 0.3 * (x - xp) + w * (r - A @ x), **tol)
+    xx = T(x)
+    with pytest.raises(FneError):
+        Ad.cheb(xx, T(xp), T(r), T(w), 0.5, xx)
+
+
+@pytest.mark.parametrize("kind", ["he", "slope"])
+def test_chebyshev_smoothed_multigrid_cg(require_gpu, kind):
+    """CG with the Chebyshev-smoothed multigrid (mg_levels_ksp_type chebyshev, 2 steps) reaches
+    the direct solution; its graph replays equal the eager cycle; iteration counts of both
+    smoothers are printed."""
+    import torch
+
+    from fenics_nonlinear_energies_amd.problems.slope_stability import SlopeStabilityReorderedElementAssembler
+    from tests.test_gpu_mc import _state
+    from tests.test_oracle_mc import mc_params
+
+    its = {}
+    for smoother in ("richardson", "chebyshev"):
+        opts = {"mg_levels_ksp_type": smoother, "mg_levels_ksp_max_it": 2}
+        if kind == "he":
+            p, adj, u0 = build_beam_problem(40, 4, 4)
+            asm = HEReorderedElementAssembler(p, None, adj, ksp_type="cg", pc_type="gamg", pc_options=opts)
+            x0 = asm.create_vec((u0 + 1e-5 * np.random.default_rng(0).standard_normal(u0.size))[asm.layout.perm])
+        else:
+            p, adj = mc_params(h=1.0, degree=4, lam=1.0)
+            asm = SlopeStabilityReorderedElementAssembler(p, None, adj, hessian_precision="fp64", ksp_type="cg",
+                                                          pc_type="mg", pc_options=opts)
+            x0 = asm.create_vec(_state(p, asm.layout, 11)[asm.layout.perm])
+        asm.assemble_hessian(x0)
+        b = asm.create_vec(np.random.default_rng(1).standard_normal(asm.layout.n_free))
+        ksp = asm.ksp
+        ksp.setOperators(asm.A)
+        ksp.setTolerances(rtol=1e-10, max_it=5000)
+        ksp.setUp()
+        assert ksp._amg.smoother == ("chebyshev" if smoother == "chebyshev" else "jacobi")
+        x = b.duplicate()
+        ksp.solve(b, x)
+        assert ksp.getConvergedReason() > 0
+        its[smoother] = ksp.getIterationNumber()
+        ref = spla.spsolve(asm.A.to_scipy().tocsc(), b.array)
+        assert np.linalg.norm(x.array - ref) <= 1e-6 * np.linalg.norm(ref)
+        pc = ksp._amg
+        r = torch.as_tensor(np.random.default_rng(2).standard_normal(asm.layout.n_free)).cuda()
+        eager = pc._cycle(0, r.clone())
+        z = torch.empty_like(r)
+        pc.apply(r, z)
+        torch.cuda.synchronize()
+        assert torch.equal(z, eager)
+        asm.cleanup()
+    print(kind, "CG iterations", its, flush=True)
