@@ -4,11 +4,14 @@ Workload: Ginzburg-Landau, eps = 0.01 on [-1,1]^2, structured 1024x1024 grid of 
 triangles (2,097,152 elements, 1,046,529 free DOFs, 7,317,521 CSR nonzeros), initial
 guess 0.5 sin(pi x) sin(pi y) + U[-1e-3, 1e-3] (seed 0).  One step = one Newton-step
 assembly: energy + gradient + CSR Hessian from one fused staged-patch kernel plus the energy
-finalisation kernel (Hessian values FP32, element math, energy and gradient FP64).  Inputs are
-resident in HBM before timing starts.
+finalisation kernel (Hessian values FP32, element math, energy and gradient FP64).  The
+kernel reads each triangle's geometry (dvx, dvy, vol) from per-element arrays
+(``--element-geometry nodes`` evaluates it from the vertex coordinates instead, bit for bit the
+same values).  Inputs are resident in HBM before timing starts.
 
 Beside the headline the line carries: ``headline_fp64`` (the same workload with FP64 Hessian
-values), ``secondary`` (assembly and SpMV of the P4 Mohr-Coulomb slope at 1.57M DOFs and of the
+values), ``headline_geometry_nodes`` (the FP32 workload with the geometry evaluated from the
+vertex coordinates), ``secondary`` (assembly and SpMV of the P4 Mohr-Coulomb slope at 1.57M DOFs and of the
 Neo-Hookean tet beams at 61,440 and 491,520 elements), ``time_to_tol`` (the metric's second
 half: Newton time to tolerance on the Neo-Hookean load path, the reference's solver profile in
 FP64 against FP32 Hessians from the learned warm start, network trained in the same run),
@@ -40,6 +43,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1024, help="cells per side")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--element-geometry", default="arrays", choices=["auto", "nodes", "arrays"],
+                    help="P1 element geometry from the vertex coordinates (auto: when they reproduce the "
+                         "geometry arrays bit for bit) or from per-element arrays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=256)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
@@ -136,10 +142,20 @@ def _time_ms(fn, reps=20):
     return a.elapsed_time(b) / reps
 
 
-def headline_fp64(n_cells: int, steps: int = 50) -> dict:
-    """The headline workload with FP64 Hessian values (the reference's precision): elem/s and the
-    HBM fraction of the same fused kernel, timed like the headline."""
-    import numpy as np
+def p1_algorithmic_bytes(asm, n_nodes: int, s_h: int) -> int:
+    """Bytes one fused P1 assembly must move: connectivity (3 x 4 B per element), the element
+    geometry (7 FP64 per element, or 16 B of vertex coordinates per node when the kernels
+    evaluate it from the coordinates), nodal state (8 B per node), gradient write (8 B per free
+    DOF) and CSR values write (s_h per nonzero)."""
+    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
+    geo = 16 * n_nodes if asm.node_geometry else 8 * 7 * ne
+    return int(ne * 4 * 3 + geo + 8 * n_nodes + 8 * nf + s_h * nnz)
+
+
+def headline_variant(n_cells: int, precision: str, element_geometry: str, steps: int = 50) -> dict:
+    """The headline workload in another form (FP64 Hessian values, the reference's precision;
+    or geometry read from per-element arrays): elem/s and the HBM fraction of the same fused
+    kernel, timed like the headline."""
     import torch
 
     from fenics_nonlinear_energies_amd.problems.ginzburg_landau import (
@@ -149,15 +165,18 @@ def headline_fp64(n_cells: int, steps: int = 50) -> dict:
     )
 
     params, adj = build_gl_problem(n_cells)
-    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision="fp64")
+    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision=precision,
+                                      element_geometry=element_geometry)
     x = asm.create_vec(sine_noise_init(params, 0)[asm.layout.perm])
     g = x.duplicate()
     ms = _time_ms(lambda: asm.energy_grad_hessian_device(x.t, g.t), reps=steps)
-    ne, nnz, nf = asm.n_elem, asm.layout.nnz, asm.layout.n_free
-    b_asm = ne * (4 * 3 + 8 * 7) + 8 * params["nodes"].shape[0] + 8 * nf + 8 * nnz
+    ne = asm.n_elem
+    b_asm = p1_algorithmic_bytes(asm, params["nodes"].shape[0], 4 if precision == "fp32" else 8)
+    geo = "vertex coordinates" if asm.node_geometry else "per-element geometry arrays"
     asm.cleanup()
     torch.cuda.synchronize()
-    return {"workload": f"Ginzburg-Landau {n_cells}x{n_cells} P1, fused energy + gradient + Hessian, Hessian fp64",
+    return {"workload": f"Ginzburg-Landau {n_cells}x{n_cells} P1, fused energy + gradient + Hessian, "
+                        f"Hessian {precision}, geometry from {geo}",
             "elem_per_s": ne / (ms * 1e-3), "assembly_ms": ms, "algorithmic_bytes": int(b_asm),
             "achieved_GBs": b_asm / (ms * 1e-3) / 1e9, "frac": b_asm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
@@ -259,7 +278,8 @@ def main():
 
     torch.cuda.set_device(0)
     params, adj = build_gl_problem(args.n)
-    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision=args.precision)
+    asm = GLReorderedElementAssembler(params, None, adj, hessian_precision=args.precision,
+                                      element_geometry=args.element_geometry)
     u = sine_noise_init(params, 0)
     x = asm.create_vec(u[asm.layout.perm])
     g = x.duplicate()
@@ -310,14 +330,13 @@ def main():
     nf = asm.layout.n_free
     nt = params["nodes"].shape[0]
     s_h = 4 if args.precision == "fp32" else 8
-    # algorithmic bytes of one assembly: connectivity + FP64 geometry (7/elem) + nodal state +
-    # gradient write + CSR values write
-    b_asm = ne * (4 * 3 + 8 * 7) + 8 * nt + 8 * nf + s_h * nnz
+    b_asm = p1_algorithmic_bytes(asm, nt, s_h)
     b_spmv = nnz * (s_h + 4) + 4 * (nf + 1) + 8 * nf + 8 * nf
     achieved = b_asm / (kern_ms * 1e-3) / 1e9
     traffic = None
     # HBM bytes per launch of the assembly kernel from the committed FETCH_SIZE/WRITE_SIZE summary
-    tfile = ROOT / "profiles" / f"p1_staged_gl{args.n}_{args.precision}_traffic.md"
+    geo_tag = "" if asm.node_geometry else "_geoarrays"
+    tfile = ROOT / "profiles" / f"p1_staged_gl{args.n}_{args.precision}{geo_tag}_traffic.md"
     if tfile.exists():
         for line in tfile.read_text().splitlines():
             if line.startswith("hbm_bytes_per_launch:"):
@@ -339,6 +358,7 @@ def main():
         "config": {
             "workload": f"Ginzburg-Landau eps=0.01, {args.n}x{args.n} P1 grid, one Newton-step assembly "
             f"(fused energy + gradient + Hessian), Hessian {args.precision}",
+            "element_geometry": "vertex coordinates" if asm.node_geometry else "per-element arrays",
             "n_elem": ne,
             "n_free": nf,
             "nnz": int(nnz),
@@ -362,9 +382,12 @@ def main():
             "algorithmic_bytes": int(b_spmv),
         },
     }
+    node_geo = bool(asm.node_geometry)
     asm.cleanup()
     if not args.no_fp64_line and args.precision == "fp32":
-        out["headline_fp64"] = headline_fp64(args.n)
+        out["headline_fp64"] = headline_variant(args.n, "fp64", args.element_geometry)
+        other = "arrays" if node_geo else "nodes"
+        out[f"headline_geometry_{other}"] = headline_variant(args.n, args.precision, other)
     if not args.no_secondary:
         out["secondary"] = secondary_workloads()
     if not args.no_time_to_tol:

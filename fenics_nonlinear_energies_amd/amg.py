@@ -49,10 +49,10 @@ import torch
 from . import native
 
 
-def _lanes_for(avg: float) -> int:
-    from .linalg import lanes_per_row
+def _spmv_code(prec: int, avg: float) -> int:
+    from .linalg import spmv_code
 
-    return lanes_per_row(avg)
+    return spmv_code(prec, avg)
 
 
 # fne_vcycle_op operations
@@ -71,7 +71,7 @@ class _DevMat:
         dt = torch.float32 if precision == "fp32" else torch.float64
         self.data = torch.zeros(self.nnz, dtype=dt, device=device)
         self.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
-        self.lpr = _lanes_for(self.nnz / max(1, self.n_rows))
+        self.code = _spmv_code(self.prec, self.nnz / max(1, self.n_rows))
 
     @classmethod
     def sharing_pattern(cls, indptr: torch.Tensor, indices: torch.Tensor, n_cols: int, precision: str) -> "_DevMat":
@@ -82,14 +82,14 @@ class _DevMat:
         dt = torch.float32 if precision == "fp32" else torch.float64
         m.data = torch.zeros(m.nnz, dtype=dt, device=indices.device)
         m.prec = native.FNE_FP32 if precision == "fp32" else native.FNE_FP64
-        m.lpr = _lanes_for(m.nnz / max(1, m.n_rows))
+        m.code = _spmv_code(m.prec, m.nnz / max(1, m.n_rows))
         return m
 
     def op(self, op: int, x, r, w, y_in, y: torch.Tensor) -> None:
         """Fused V-cycle operator (fne_vcycle_op): sweep / residual / prolongation-add."""
         native.check(
             native.lib().fne_vcycle_op(op, self.n_rows, native.ptr(self.indptr), native.ptr(self.indices),
-                                       native.ptr(self.data), self.prec | (self.lpr << 8), native.ptr(x),
+                                       native.ptr(self.data), self.code, native.ptr(x),
                                        native.ptr(r), native.ptr(w), native.ptr(y_in), native.ptr(y),
                                        native.stream_ptr()),
             "fne_vcycle_op",
@@ -99,7 +99,7 @@ class _DevMat:
         """Chebyshev smoothing step y = x + c (x - x_prev) + w.(r - A x) (fne_vcycle_cheb)."""
         native.check(
             native.lib().fne_vcycle_cheb(self.n_rows, native.ptr(self.indptr), native.ptr(self.indices),
-                                         native.ptr(self.data), self.prec | (self.lpr << 8), native.ptr(x),
+                                         native.ptr(self.data), self.code, native.ptr(x),
                                          native.ptr(x_prev), native.ptr(r), native.ptr(w), float(c), native.ptr(y),
                                          native.stream_ptr()),
             "fne_vcycle_cheb",
@@ -108,7 +108,7 @@ class _DevMat:
     def mult(self, x: torch.Tensor, y: torch.Tensor) -> None:
         native.check(
             native.lib().fne_spmv(self.n_rows, native.ptr(self.indptr), native.ptr(self.indices),
-                                  native.ptr(self.data), self.prec | (self.lpr << 8), native.ptr(x), native.ptr(y),
+                                  native.ptr(self.data), self.code, native.ptr(x), native.ptr(y),
                                   native.stream_ptr()),
             "fne_spmv",
         )

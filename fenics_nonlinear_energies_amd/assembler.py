@@ -88,6 +88,7 @@ class ReorderedElementAssemblerBase:
         rows_per_patch=None,
         chunk=None,
         lds_budget=None,
+        element_geometry="arrays",
         device="cuda",
     ):
         if comm is not None and int(comm.Get_size()) != 1:
@@ -113,6 +114,10 @@ class ReorderedElementAssemblerBase:
         self.assembly_backend = str(assembly_backend)
         self.hessian_precision = str(hessian_precision)
         self.assembly_mode = str(assembly_mode)
+        if element_geometry not in ("auto", "arrays", "nodes"):
+            raise ValueError(f"element_geometry must be 'auto', 'arrays' or 'nodes', not {element_geometry!r}")
+        self.element_geometry = str(element_geometry)
+        self.node_geometry = False
         self._lds_budget = None if lds_budget is None else int(lds_budget)
         self.memory_guard_total_gib = memory_guard_total_gib
         self.iter_timings: list[dict] = []
@@ -232,6 +237,40 @@ class ReorderedElementAssemblerBase:
         """(n_geo, n_elem) FP64 element data in the kernel's order."""
         raise NotImplementedError
 
+    def _geometry_from_nodes(self) -> np.ndarray | None:
+        """The element geometry as the kernels evaluate it from the vertex coordinates
+        (FNE_FLAG_NODE_GEO), same shape as ``_element_geometry``; None where the family has no
+        such form."""
+        return None
+
+    def _choose_node_geometry(self) -> bool:
+        """Whether the kernels evaluate the element geometry from the vertex coordinates:
+        "auto" when the coordinates reproduce the given geometry arrays bit for bit (then
+        both forms give identical results and the node form stores 16 bytes per vertex instead
+        of n_geo doubles per element and patch element), "nodes" requires it, "arrays" (the
+        default) never.  On the 1024^2 Ginzburg-Landau grid both forms assemble in the same
+        time (the staged kernel is bound by latency, not bytes), so the node form is a memory
+        saving: 255 MB there."""
+        if self.element_geometry == "arrays":
+            return False
+        geo_nodes = self._geometry_from_nodes()
+        same = geo_nodes is not None and np.array_equal(geo_nodes, self._element_geometry())
+        if self.element_geometry == "nodes" and not same:
+            raise ValueError("element_geometry='nodes': the vertex coordinates do not reproduce the element "
+                             "geometry arrays bit for bit (or the family has no node form)")
+        return bool(same)
+
+    def _dof_coordinates(self) -> np.ndarray:
+        """(n_free + n_total, 2) vertex coordinates per DOF code: free DOFs in the reordered
+        numbering, then every node (the Dirichlet codes n_free + node)."""
+        lay = self.layout
+        nodes = np.asarray(self.params[self.coordinate_key], dtype=np.float64)
+        t2f = lay.total_to_free_reord
+        node_of_free = np.empty(lay.n_free, dtype=np.int64)
+        free_nodes = np.flatnonzero(t2f >= 0)
+        node_of_free[t2f[free_nodes]] = free_nodes
+        return np.ascontiguousarray(np.concatenate((nodes[node_of_free], nodes), axis=0)[:, :2])
+
     def _kernel_params(self) -> list[float]:
         return []
 
@@ -319,6 +358,7 @@ class ReorderedElementAssemblerBase:
         order = plan.elem_order
         n_free = lay.n_free
         self._host_codes = np.where(elem_free >= 0, elem_free, n_free + elems)[order]
+        self.node_geometry = self._choose_node_geometry()
         self._host_geo = self._element_geometry()[:, order]
         self.n_elem = int(elems.shape[0])
         self.n_loc = int(elems.shape[1])
@@ -328,9 +368,9 @@ class ReorderedElementAssemblerBase:
         plan = self.patch_plan
         n_free = lay.n_free
         codes, geo = self._host_codes, self._host_geo
+        node_geo = bool(self.node_geometry)
         self._dev = {
             "edof": self._t(codes.T, np.int32),
-            "egeo": self._t(geo, np.float64),
             "indptr": self._t(lay.indptr, np.int32),
             "indices": self._t(lay.indices, np.int32),
             "prow": self._t(plan.prow, np.int32),
@@ -339,12 +379,17 @@ class ReorderedElementAssemblerBase:
             "inc_ptr": self._t(plan.inc_ptr, np.int32),
             "inc": self._t(plan.inc.view(np.int32), np.int32),
         }
+        if node_geo:
+            self._dev["dof_xy"] = self._t(self._dof_coordinates(), np.float64)
+        else:
+            self._dev["egeo"] = self._t(geo, np.float64)
         if plan.inc_aux is not None:
             self._dev["inc_aux"] = self._t(plan.inc_aux, np.int32)
         if plan.n_patch > 0:
             pe = plan.pelem.astype(np.int64)
             self._dev["pedof"] = self._t(codes[pe].T, np.int32)
-            self._dev["pgeo"] = self._t(geo[:, pe], np.float64)
+            if not node_geo:
+                self._dev["pgeo"] = self._t(geo[:, pe], np.float64)
             self._dev["pown"] = self._t(plan.pown, np.int32)
         if getattr(plan, "pitem", None) is not None:
             self._dev["pitem"] = self._t(np.ascontiguousarray(plan.pitem).view(np.int64), np.int64)
@@ -364,7 +409,8 @@ class ReorderedElementAssemblerBase:
         d.n_total = int(self.dirichlet_full.size)
         d.nnz = int(lay.nnz)
         d.edof = native.ptr(self._dev["edof"])
-        d.egeo = native.ptr(self._dev["egeo"])
+        d.egeo = native.ptr(self._dev.get("egeo"))
+        d.dof_xy = native.ptr(self._dev.get("dof_xy"))
         d.eweight = None
         d.indptr = native.ptr(self._dev["indptr"])
         d.indices = native.ptr(self._dev["indices"])
@@ -389,7 +435,7 @@ class ReorderedElementAssemblerBase:
             d.param[i] = float(prm[i])
         d.qdata = native.ptr(self._dev.get("qdata"))
         d.nq = int(0 if q is None else self._quadrature_points())
-        d.flags = int(self._plan_flags())
+        d.flags = int(self._plan_flags()) | (native.FNE_FLAG_NODE_GEO if node_geo else 0)
         self._desc = d
         import ctypes
 

@@ -107,24 +107,18 @@ __global__ __launch_bounds__(kLanes) void spgemm_numeric_kernel(
 //   kAddProduct: y = y_in + A x                 (prolongation added to the iterate; y may be y_in)
 enum VcOp : int { kSweepFirst = 0, kSweep = 1, kResidual = 2, kAddProduct = 3 };
 
-template <typename VT, int LPR, int OP, bool VEC>
+template <typename VT, int LPR, int OP, int FORM>
 __global__ __launch_bounds__(256) void vcycle_op_kernel(int n, const int32_t* __restrict__ indptr,
                                                         const int32_t* __restrict__ indices,
                                                         const VT* __restrict__ vals, const double* __restrict__ x,
                                                         const double* __restrict__ r, const double* __restrict__ w,
                                                         const double* y_in, double* y) {
-  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int row = gtid / LPR;
-  const int lane = gtid % LPR;
-  double acc = 0.0;
-  if (row < n) {
-    const int kb = indptr[row], ke = indptr[row + 1];
-    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, indptr[n], lane, indices, vals,
-                                    [&](int c) { return OP == kSweepFirst ? w[c] * r[c] : x[c]; });
-  }
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-  if (row < n && lane == 0) {
+  __shared__ double prod[FORM == 2 ? kStreamCap : 1];
+  int row;
+  bool leader;
+  const double acc = csr_form_dot<VT, LPR, FORM>(
+      n, indptr, indices, vals, [&](int c) { return OP == kSweepFirst ? w[c] * r[c] : x[c]; }, prod, row, leader);
+  if (leader) {
     double out;
     if (OP == kSweepFirst) {
       const double x0 = w[row] * r[row];
@@ -140,46 +134,56 @@ __global__ __launch_bounds__(256) void vcycle_op_kernel(int n, const int32_t* __
   }
 }
 
+// form of a row kernel's CSR pass: 2 CSR-stream when asked for (precision bits 16..19) and the
+// arrays are aligned, else 1 / 0 lanes per row with / without grouped loads
+int csr_kernel_form(int32_t precision, const void* indices, const void* vals, int lpr) {
+  if (csr_form_of(precision) == kCsrFormStream && csr_stream_ok(indices, vals)) return 2;
+  return csr_use_vec(indices, vals, lpr) ? 1 : 0;
+}
+
 template <typename VT, int OP>
 void launch_vcycle_op(int n, const int32_t* indptr, const int32_t* indices, const VT* vals, const double* x,
-                      const double* r, const double* w, const double* y_in, double* y, int lpr, hipStream_t s) {
+                      const double* r, const double* w, const double* y_in, double* y, int lpr, int form,
+                      hipStream_t s) {
   auto go = [&](auto kern, int L) {
     const long long threads = (long long)n * L;
     const int nb = int((threads + 255) / 256);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, n, indptr, indices, vals, x, r, w, y_in, y);
   };
-  if (csr_use_vec(indices, vals, lpr)) {
-    switch (lpr) {
-      case 1: go(vcycle_op_kernel<VT, 1, OP, true>, 1); break;
-      case 2: go(vcycle_op_kernel<VT, 2, OP, true>, 2); break;
-      case 4: go(vcycle_op_kernel<VT, 4, OP, true>, 4); break;
-      case 8: go(vcycle_op_kernel<VT, 8, OP, true>, 8); break;
-      case 16: go(vcycle_op_kernel<VT, 16, OP, true>, 16); break;
-      case 32: go(vcycle_op_kernel<VT, 32, OP, true>, 32); break;
-      default: go(vcycle_op_kernel<VT, 64, OP, true>, 64); break;
-    }
-  } else {
-    switch (lpr) {
-      case 1: go(vcycle_op_kernel<VT, 1, OP, false>, 1); break;
-      case 2: go(vcycle_op_kernel<VT, 2, OP, false>, 2); break;
-      case 4: go(vcycle_op_kernel<VT, 4, OP, false>, 4); break;
-      case 8: go(vcycle_op_kernel<VT, 8, OP, false>, 8); break;
-      case 16: go(vcycle_op_kernel<VT, 16, OP, false>, 16); break;
-      case 32: go(vcycle_op_kernel<VT, 32, OP, false>, 32); break;
-      default: go(vcycle_op_kernel<VT, 64, OP, false>, 64); break;
-    }
+#define VC_L(L)                                                    \
+  case L:                                                          \
+    if (form == 2) go(vcycle_op_kernel<VT, L, OP, 2>, L);          \
+    else if (form == 1) go(vcycle_op_kernel<VT, L, OP, 1>, L);     \
+    else go(vcycle_op_kernel<VT, L, OP, 0>, L);                    \
+    break;
+  switch (lpr) {
+    VC_L(1)
+    VC_L(2)
+    VC_L(4)
+    VC_L(8)
+    VC_L(16)
+    VC_L(32)
+    default:
+      if (form == 2) go(vcycle_op_kernel<VT, 64, OP, 2>, 64);
+      else if (form == 1) go(vcycle_op_kernel<VT, 64, OP, 1>, 64);
+      else go(vcycle_op_kernel<VT, 64, OP, 0>, 64);
   }
+#undef VC_L
 }
 
 template <typename VT>
 int vcycle_op_dispatch(int op, int n, const int32_t* indptr, const int32_t* indices, const VT* vals,
                        const double* x, const double* r, const double* w, const double* y_in, double* y, int lpr,
-                       hipStream_t s) {
+                       int form, hipStream_t s) {
   switch (op) {
-    case kSweepFirst: launch_vcycle_op<VT, kSweepFirst>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, s); break;
-    case kSweep: launch_vcycle_op<VT, kSweep>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, s); break;
-    case kResidual: launch_vcycle_op<VT, kResidual>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, s); break;
-    case kAddProduct: launch_vcycle_op<VT, kAddProduct>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, s); break;
+    case kSweepFirst:
+      launch_vcycle_op<VT, kSweepFirst>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, form, s);
+      break;
+    case kSweep: launch_vcycle_op<VT, kSweep>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, form, s); break;
+    case kResidual: launch_vcycle_op<VT, kResidual>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, form, s); break;
+    case kAddProduct:
+      launch_vcycle_op<VT, kAddProduct>(n, indptr, indices, vals, x, r, w, y_in, y, lpr, form, s);
+      break;
     default:
       fne_set_error("fne_vcycle_op: unknown op");
       return FNE_EINVAL;
@@ -336,12 +340,13 @@ extern "C" int fne_vcycle_op(int32_t op, int32_t n_rows, const int32_t* indptr, 
     return FNE_EINVAL;
   }
   const int prec = precision & 0xff;
+  const int form = csr_kernel_form(precision, indices, vals, lpr);
   int rc;
   if (prec == FNE_FP64)
-    rc = vcycle_op_dispatch<double>(op, n_rows, indptr, indices, (const double*)vals, x, r, w, y_in, y, lpr,
+    rc = vcycle_op_dispatch<double>(op, n_rows, indptr, indices, (const double*)vals, x, r, w, y_in, y, lpr, form,
                                     (hipStream_t)stream);
   else if (prec == FNE_FP32)
-    rc = vcycle_op_dispatch<float>(op, n_rows, indptr, indices, (const float*)vals, x, r, w, y_in, y, lpr,
+    rc = vcycle_op_dispatch<float>(op, n_rows, indptr, indices, (const float*)vals, x, r, w, y_in, y, lpr, form,
                                    (hipStream_t)stream);
   else {
     fne_set_error("fne_vcycle_op: precision must be FNE_FP64 or FNE_FP32");
@@ -357,23 +362,18 @@ extern "C" int fne_vcycle_op(int32_t op, int32_t n_rows, const int32_t* indptr, 
 // iteration for D^-1 A on [a, b] (theta = (a + b) / 2, delta = (b - a) / 2); x_prev may be null
 // when c = 0.  y may alias x_prev (each row reads its own x_prev entry before writing it).
 namespace {
-template <typename VT, int LPR, bool VEC>
+template <typename VT, int LPR, int FORM>
 __global__ __launch_bounds__(256) void vcycle_cheb_kernel(int n, const int32_t* __restrict__ indptr,
                                                           const int32_t* __restrict__ indices,
                                                           const VT* __restrict__ vals, const double* __restrict__ x,
                                                           const double* x_prev, const double* __restrict__ r,
                                                           const double* __restrict__ w, double c, double* y) {
-  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int row = gtid / LPR;
-  const int lane = gtid % LPR;
-  double acc = 0.0;
-  if (row < n) {
-    const int kb = indptr[row], ke = indptr[row + 1];
-    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, indptr[n], lane, indices, vals, [&](int col) { return x[col]; });
-  }
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-  if (row < n && lane == 0) {
+  __shared__ double prod[FORM == 2 ? kStreamCap : 1];
+  int row;
+  bool leader;
+  const double acc =
+      csr_form_dot<VT, LPR, FORM>(n, indptr, indices, vals, [&](int col) { return x[col]; }, prod, row, leader);
+  if (leader) {
     const double xr = x[row];
     double out = xr + w[row] * (r[row] - acc);
     if (x_prev) out += c * (xr - x_prev[row]);
@@ -384,17 +384,17 @@ __global__ __launch_bounds__(256) void vcycle_cheb_kernel(int n, const int32_t* 
 template <typename VT>
 int launch_vcycle_cheb(int n, const int32_t* indptr, const int32_t* indices, const VT* vals, const double* x,
                        const double* x_prev, const double* r, const double* w, double c, double* y, int lpr,
-                       hipStream_t s) {
+                       int form, hipStream_t s) {
   auto go = [&](auto kern, int L) {
     const long long threads = (long long)n * L;
     const int nb = int((threads + 255) / 256);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, n, indptr, indices, vals, x, x_prev, r, w, c, y);
   };
-  const bool vec = csr_use_vec(indices, vals, lpr);
 #define CHEB_L(L)                                                   \
   case L:                                                           \
-    if (vec) go(vcycle_cheb_kernel<VT, L, true>, L);                 \
-    else go(vcycle_cheb_kernel<VT, L, false>, L);                    \
+    if (form == 2) go(vcycle_cheb_kernel<VT, L, 2>, L);              \
+    else if (form == 1) go(vcycle_cheb_kernel<VT, L, 1>, L);         \
+    else go(vcycle_cheb_kernel<VT, L, 0>, L);                        \
     break;
   switch (lpr) {
     CHEB_L(1)
@@ -404,8 +404,9 @@ int launch_vcycle_cheb(int n, const int32_t* indptr, const int32_t* indices, con
     CHEB_L(16)
     CHEB_L(32)
     default:
-      if (vec) go(vcycle_cheb_kernel<VT, 64, true>, 64);
-      else go(vcycle_cheb_kernel<VT, 64, false>, 64);
+      if (form == 2) go(vcycle_cheb_kernel<VT, 64, 2>, 64);
+      else if (form == 1) go(vcycle_cheb_kernel<VT, 64, 1>, 64);
+      else go(vcycle_cheb_kernel<VT, 64, 0>, 64);
   }
 #undef CHEB_L
   FNE_LAUNCH_CHECK();
@@ -436,11 +437,14 @@ extern "C" int fne_vcycle_cheb(int32_t n_rows, const int32_t* indptr, const int3
     return FNE_EINVAL;
   }
   const int prec = precision & 0xff;
+  const int form = csr_kernel_form(precision, indices, vals, lpr);
   hipStream_t s = (hipStream_t)stream;
   if (prec == FNE_FP64)
-    return launch_vcycle_cheb<double>(n_rows, indptr, indices, (const double*)vals, x, x_prev, r, w, c, y, lpr, s);
+    return launch_vcycle_cheb<double>(n_rows, indptr, indices, (const double*)vals, x, x_prev, r, w, c, y, lpr, form,
+                                      s);
   if (prec == FNE_FP32)
-    return launch_vcycle_cheb<float>(n_rows, indptr, indices, (const float*)vals, x, x_prev, r, w, c, y, lpr, s);
+    return launch_vcycle_cheb<float>(n_rows, indptr, indices, (const float*)vals, x, x_prev, r, w, c, y, lpr, form,
+                                     s);
   fne_set_error("fne_vcycle_cheb: precision must be FNE_FP64 or FNE_FP32");
   return FNE_EINVAL;
 }

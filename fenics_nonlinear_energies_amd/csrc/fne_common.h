@@ -189,6 +189,111 @@ __host__ __forceinline__ int csr_lanes_per_row(double avg_row_len) {
   return l;
 }
 
+// ---------------------------------------------------------------------------------------------
+// CSR-stream form (Greathouse and Daga, "Efficient sparse matrix-vector multiplication on GPUs
+// using the CSR storage format", SC'14): a 256-thread workgroup takes R = 256 / LPR consecutive
+// rows [r0, r1) and streams their entries [indptr[r0], indptr[r1]) through LDS in chunks of
+// kStreamCap, each thread one 16-byte group of four column indices and four values per pass --
+// independent, coalesced loads, many in flight -- as FP64 products vals[k] * xf(indices[k]);
+// then LPR lanes per row sum their row's products from LDS (lane l: entries kb + l, kb + l + LPR,
+// ... in increasing k, chunk by chunk).  The caller joins the LPR lanes with a fixed xor tree, so
+// the result is bitwise reproducible.  Every thread of the workgroup must call it (barriers);
+// indices and vals must be 16-byte aligned (csr_stream_ok); prod is kStreamCap doubles of LDS.
+// ---------------------------------------------------------------------------------------------
+constexpr int kStreamCap = 2048;
+
+template <typename VT, int LPR, typename XF>
+__device__ __forceinline__ double csr_stream_dot(int r0, int r1, int nnz, const int32_t* __restrict__ indptr,
+                                                 const int32_t* __restrict__ indices, const VT* __restrict__ vals,
+                                                 XF xf, double* prod) {
+  const int tid = threadIdx.x;
+  const int row = r0 + tid / LPR, lane = tid % LPR;
+  const bool has = row < r1;
+  const int base = indptr[r0], end = indptr[r1];
+  const int kb = has ? indptr[row] : 0, ke = has ? indptr[row + 1] : 0;
+  double acc = 0.0;
+  for (int c0 = base & ~3; c0 < end; c0 += kStreamCap) {
+#pragma unroll
+    for (int j = 4 * tid; j < kStreamCap; j += 4 * 256) {
+      const int g = c0 + j;
+      double t[4] = {0.0, 0.0, 0.0, 0.0};
+      if (g < end) {
+        if (g + 4 <= nnz) {
+          const int4 c4 = *reinterpret_cast<const int4*>(indices + g);
+          double v[4];
+          CsrGroup4<VT>::load(vals + g, v);
+          const double x0 = xf(c4.x), x1 = xf(c4.y), x2 = xf(c4.z), x3 = xf(c4.w);
+          const double p[4] = {v[0] * x0, v[1] * x1, v[2] * x2, v[3] * x3};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) t[q] = (g + q >= base && g + q < end) ? p[q] : 0.0;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (g + q >= base && g + q < end) t[q] = double(vals[g + q]) * xf(indices[g + q]);
+        }
+      }
+      reinterpret_cast<double2*>(prod + j)[0] = make_double2(t[0], t[1]);
+      reinterpret_cast<double2*>(prod + j)[1] = make_double2(t[2], t[3]);
+    }
+    __syncthreads();
+    if (has) {
+      const int lo = kb > c0 ? kb : c0;
+      const int hi = ke < c0 + kStreamCap ? ke : c0 + kStreamCap;
+      for (int k = lo + lane; k < hi; k += LPR) acc += prod[k - c0];
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
+// One CSR row pass of a 256-thread row kernel in either form, lanes joined by the fixed xor
+// tree: FORM 0 lanes per row (row r = thread / LPR over the grid), 1 the same with grouped
+// loads, 2 CSR-stream (rows blockIdx * 256 / LPR onward, prod = kStreamCap doubles of LDS).
+// Sets row (n when the thread has none) and leader (the thread that writes the row's result).
+// All threads of the workgroup must call it.  Workgroups: ceil(n * LPR / 256) in every form.
+template <typename VT, int LPR, int FORM, typename XF>
+__device__ __forceinline__ double csr_form_dot(int n, const int32_t* __restrict__ indptr,
+                                               const int32_t* __restrict__ indices, const VT* __restrict__ vals,
+                                               XF xf, double* prod, int& row, bool& leader) {
+  double acc = 0.0;
+  int lane;
+  if constexpr (FORM == 2) {
+    constexpr int R = 256 / LPR;
+    const int r0 = xcd_remap(blockIdx.x, gridDim.x) * R;
+    const int r1 = r0 + R < n ? r0 + R : n;
+    acc = csr_stream_dot<VT, LPR>(r0, r1, indptr[n], indptr, indices, vals, xf, prod);
+    row = r0 + int(threadIdx.x) / LPR;
+    lane = int(threadIdx.x) % LPR;
+    if (row >= r1) row = n;
+  } else {
+    const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    row = gtid / LPR;
+    lane = gtid % LPR;
+    if (row < n) acc = csr_row_dot<VT, LPR, FORM == 1>(indptr[row], indptr[row + 1], indptr[n], lane, indices, vals, xf);
+    else row = n;
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  leader = row < n && lane == 0;
+  return acc;
+}
+
+// lanes per row of the CSR-stream form: about 8 entries per lane (rows per workgroup 256 / lanes)
+__host__ __forceinline__ int csr_stream_lanes(double avg_row_len) {
+  int l = 1;
+  while (l < 64 && 2.0 * l <= avg_row_len / 8.0) l *= 2;
+  return l;
+}
+
+__host__ __forceinline__ bool csr_stream_ok(const void* indices, const void* vals) {
+  return (reinterpret_cast<uintptr_t>(indices) & 15u) == 0 && (reinterpret_cast<uintptr_t>(vals) & 15u) == 0;
+}
+
+// the SpMV-form field of the precision argument of fne_spmv / fne_vcycle_op / fne_vcycle_cheb
+// (bits 16..19): 0 = lanes per row, 1 = CSR-stream
+constexpr int kCsrFormStream = 1;
+__host__ __forceinline__ int csr_form_of(int32_t precision) { return (precision >> 16) & 0xF; }
+
 // gather one element value: free DOF from x, constrained DOF from the Dirichlet vector
 __device__ __forceinline__ double gather_dof(int c, int n_free, const double* __restrict__ x,
                                              const double* __restrict__ dvals) {

@@ -50,6 +50,24 @@ __global__ __launch_bounds__(kBlock) void spmv_kernel(int n, const int32_t* __re
   if (row < n && lane == 0) y[row] = acc;
 }
 
+// y = A x in the CSR-stream form (csr_stream_dot): R = 256 / LPR rows per workgroup.
+template <typename VT, int LPR>
+__global__ __launch_bounds__(kBlock) void spmv_stream_kernel(int n, const int32_t* __restrict__ indptr,
+                                                             const int32_t* __restrict__ indices,
+                                                             const VT* __restrict__ vals,
+                                                             const double* __restrict__ x,
+                                                             double* __restrict__ y) {
+  __shared__ double prod[kStreamCap];
+  constexpr int R = kBlock / LPR;
+  const int r0 = xcd_remap(blockIdx.x, gridDim.x) * R;
+  const int r1 = min(r0 + R, n);
+  double acc = csr_stream_dot<VT, LPR>(r0, r1, indptr[n], indptr, indices, vals, [&](int c) { return x[c]; }, prod);
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  const int row = r0 + int(threadIdx.x) / LPR;
+  if (row < r1 && int(threadIdx.x) % LPR == 0) y[row] = acc;
+}
+
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void diag_kernel(int n, const int32_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ indices,
@@ -83,6 +101,27 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* __
     acc_add(s, c, a[i] * b[i]);
   block_reduce_pair(s, c);
   if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s, c);
+}
+
+template <typename VT>
+int launch_spmv_stream(int n, const int32_t* indptr, const int32_t* indices, const VT* vals,
+                       const double* x, double* y, hipStream_t stream, int lpr) {
+  auto go = [&](auto kern, int L) {
+    const int rows = kBlock / L;
+    const int nb = int((static_cast<long long>(n) + rows - 1) / rows);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, stream, n, indptr, indices, vals, x, y);
+  };
+  switch (lpr) {
+    case 1: go(spmv_stream_kernel<VT, 1>, 1); break;
+    case 2: go(spmv_stream_kernel<VT, 2>, 2); break;
+    case 4: go(spmv_stream_kernel<VT, 4>, 4); break;
+    case 8: go(spmv_stream_kernel<VT, 8>, 8); break;
+    case 16: go(spmv_stream_kernel<VT, 16>, 16); break;
+    case 32: go(spmv_stream_kernel<VT, 32>, 32); break;
+    default: go(spmv_stream_kernel<VT, 64>, 64); break;
+  }
+  FNE_LAUNCH_CHECK();
+  return FNE_OK;
 }
 
 template <typename VT>
@@ -124,8 +163,18 @@ int validate_desc(const fne_plan_desc* d) {
     fne_set_error("negative size in plan descriptor");
     return FNE_EINVAL;
   }
-  if (d->n_elem > 0 && (!d->edof || !d->egeo)) {
-    fne_set_error("plan descriptor is missing element arrays");
+  const bool node_geo = (d->flags & FNE_FLAG_NODE_GEO) != 0;
+  if (d->n_elem > 0 && (!d->edof || (node_geo ? !d->dof_xy : !d->egeo))) {
+    fne_set_error(node_geo ? "plan descriptor with FNE_FLAG_NODE_GEO is missing edof or dof_xy"
+                           : "plan descriptor is missing element arrays");
+    return FNE_EINVAL;
+  }
+  if (node_geo && d->kind != FNE_KIND_PLAPLACE && d->kind != FNE_KIND_GINZBURG_LANDAU) {
+    fne_set_error("FNE_FLAG_NODE_GEO is defined for the P1 scalar families only");
+    return FNE_EINVAL;
+  }
+  if (node_geo && (reinterpret_cast<uintptr_t>(d->dof_xy) & 15u) != 0) {
+    fne_set_error("dof_xy must be 16-byte aligned");
     return FNE_EINVAL;
   }
   switch (d->kind) {
@@ -349,6 +398,14 @@ int fne_spmv(int32_t n_rows, const int32_t* indptr, const int32_t* indices, cons
     return FNE_EINVAL;
   }
   const int prec = precision & 0xFF;
+  // bits 16..19: 1 = CSR-stream form (rows per workgroup 256 / lanes per row), when aligned
+  if (csr_form_of(precision) == kCsrFormStream && csr_stream_ok(indices, vals)) {
+    if (prec == FNE_FP32)
+      return launch_spmv_stream<float>(n_rows, indptr, indices, (const float*)vals, x, y, (hipStream_t)stream, lpr);
+    if (prec == FNE_FP64)
+      return launch_spmv_stream<double>(n_rows, indptr, indices, (const double*)vals, x, y, (hipStream_t)stream,
+                                        lpr);
+  }
   if (prec == FNE_FP32)
     return launch_spmv<float>(n_rows, indptr, indices, (const float*)vals, x, y, (hipStream_t)stream,
                               lpr);

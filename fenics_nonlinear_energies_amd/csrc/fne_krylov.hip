@@ -125,8 +125,10 @@ __global__ __launch_bounds__(1024) void cg_init_finalize(CgParams P, const doubl
 }
 
 // p_new = z + ratio p_old; w = A p_new (on the fly from z and p_old); partials of p_new.w
-// (trust mode also p_new.p_new and x.p_new, at part + gridDim.x and part + 2 gridDim.x)
-template <typename VT, int LPR, bool VEC>
+// (trust mode also p_new.p_new and x.p_new, at part + gridDim.x and part + 2 gridDim.x).
+// FORM: 0 lanes per row, 1 lanes per row with grouped loads, 2 CSR-stream (256 / LPR rows per
+// workgroup)
+template <typename VT, int LPR, int FORM>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* __restrict__ vals,
                                                          const double* __restrict__ z,
                                                          const double* __restrict__ p_old,
@@ -134,21 +136,15 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgParams P, const VT* _
                                                          const double* __restrict__ x,
                                                          const double* __restrict__ st,
                                                          double2* __restrict__ part) {
-  if (st[kDone] != 0.0) return;
+  if (st[kDone] != 0.0) return;  // uniform over the grid
   const double ratio = st[kRatio];
-  const int gtid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int row = gtid / LPR;
-  const int lane = gtid % LPR;
-  double acc = 0.0;
-  if (row < P.n) {
-    const int kb = P.indptr[row], ke = P.indptr[row + 1];
-    acc = csr_row_dot<VT, LPR, VEC>(kb, ke, P.indptr[P.n], lane, P.indices, vals,
-                                    [&](int c) { return z[c] + ratio * p_old[c]; });
-  }
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+  __shared__ double prod[FORM == 2 ? kStreamCap : 1];
+  int row;
+  bool leader;
+  const double acc = csr_form_dot<VT, LPR, FORM>(
+      P.n, P.indptr, P.indices, vals, [&](int c) { return z[c] + ratio * p_old[c]; }, prod, row, leader);
   double s = 0.0, cc = 0.0, s2 = 0.0, c2 = 0.0, s3 = 0.0, c3 = 0.0;
-  if (row < P.n && lane == 0) {
+  if (leader) {
     const double pn = z[row] + ratio * p_old[row];
     p_new[row] = pn;
     w[row] = acc;
@@ -339,34 +335,64 @@ __global__ __launch_bounds__(kBlock) void cg_dots_kernel(int n, int trust, const
 
 constexpr int kVecBlocks = 1024;
 
-int lanes_for(int n, int64_t nnz) { return csr_lanes_per_row(n > 0 ? double(nnz) / n : 1.0); }
+// the SpMV pass of one CG iteration: CSR-stream when the arrays are 16-byte aligned (lanes per
+// row from csr_stream_lanes), else lanes per row (csr_lanes_per_row)
+struct CgSpmvShape {
+  int form;  // 0, 1, 2 as cg_spmv_kernel's FORM
+  int lpr;
+  int blocks;
+};
+
+CgSpmvShape cg_spmv_shape(int n, int64_t nnz, const void* indices, const void* vals) {
+  const double avg = n > 0 ? double(nnz) / n : 1.0;
+  CgSpmvShape sh;
+  if (csr_stream_ok(indices, vals)) {
+    sh.form = 2;
+    sh.lpr = csr_stream_lanes(avg);
+  } else {
+    sh.lpr = csr_lanes_per_row(avg);
+    sh.form = csr_use_vec(indices, vals, sh.lpr) ? 1 : 0;
+  }
+  sh.blocks = int((int64_t(n) * sh.lpr + kBlock - 1) / kBlock);
+  return sh;
+}
 
 template <typename VT, int LPR>
-void launch_cg_spmv(const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
+void launch_cg_spmv(int form, const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
                     double* w, const double* x, const double* st, double2* part, int nbs, hipStream_t s) {
-  if (csr_use_vec(P.indices, vals, LPR))
-    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, true>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w,
-                       x, st, part);
+  if (form == 2)
+    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, 2>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, x,
+                       st, part);
+  else if (form == 1)
+    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, 1>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, x,
+                       st, part);
   else
-    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, false>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w,
-                       x, st, part);
+    hipLaunchKernelGGL((cg_spmv_kernel<VT, LPR, 0>), dim3(nbs), dim3(kBlock), 0, s, P, vals, z, p_old, p_new, w, x,
+                       st, part);
 }
 
 template <typename VT>
-void cg_spmv(int lpr, const CgParams& P, const VT* vals, const double* z, const double* p_old, double* p_new,
-             double* w, const double* x, const double* st, double2* part, int nbs, hipStream_t s) {
-  switch (lpr) {
-    case 1: launch_cg_spmv<VT, 1>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    case 2: launch_cg_spmv<VT, 2>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    case 4: launch_cg_spmv<VT, 4>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    case 8: launch_cg_spmv<VT, 8>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    case 16: launch_cg_spmv<VT, 16>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    case 32: launch_cg_spmv<VT, 32>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
-    default: launch_cg_spmv<VT, 64>(P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+void cg_spmv(const CgSpmvShape& sh, const CgParams& P, const VT* vals, const double* z, const double* p_old,
+             double* p_new, double* w, const double* x, const double* st, double2* part, hipStream_t s) {
+  const int f = sh.form, nbs = sh.blocks;
+  switch (sh.lpr) {
+    case 1: launch_cg_spmv<VT, 1>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 2: launch_cg_spmv<VT, 2>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 4: launch_cg_spmv<VT, 4>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 8: launch_cg_spmv<VT, 8>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 16: launch_cg_spmv<VT, 16>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    case 32: launch_cg_spmv<VT, 32>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
+    default: launch_cg_spmv<VT, 64>(f, P, vals, z, p_old, p_new, w, x, st, part, nbs, s); break;
   }
 }
 
-int64_t spmv_blocks(int n, int lpr) { return (int64_t(n) * lpr + kBlock - 1) / kBlock; }
+// workgroups of the SpMV pass, the larger of the two forms (the workspace is sized before the
+// arrays' alignment is known)
+int64_t spmv_blocks_max(int n, int64_t nnz) {
+  const double avg = n > 0 ? double(nnz) / n : 1.0;
+  const int64_t l = std::max(csr_lanes_per_row(avg), csr_stream_lanes(avg));
+  return (int64_t(n) * l + kBlock - 1) / kBlock;
+}
 
 }  // namespace
 
@@ -374,8 +400,7 @@ extern "C" {
 
 int64_t fne_pcg_workspace_bytes(int32_t n, int64_t nnz) {
   if (n < 0 || nnz < 0) return FNE_EINVAL;
-  const int lpr = lanes_for(n, nnz);
-  int64_t nparts = 3 * spmv_blocks(n, lpr);  // p.w, and in trust mode p.p and x.p
+  int64_t nparts = 3 * spmv_blocks_max(n, nnz);  // p.w, and in trust mode p.p and x.p
   if (nparts < 2 * kVecBlocks) nparts = 2 * kVecBlocks;
   return int64_t(sizeof(double)) * (5 * int64_t(n) + kStateLen) + int64_t(sizeof(double2)) * nparts;
 }
@@ -408,8 +433,8 @@ int pcg_run(const char* name, int trust, double radius, int32_t n, const int32_t
     return FNE_EINVAL;
   }
   if (check_every < 1) check_every = 1;
-  const int lpr = lanes_for(n, nnz64);
-  const int nbs = int(spmv_blocks(n, lpr));
+  const CgSpmvShape sh = cg_spmv_shape(n, nnz64, indices, vals);
+  const int nbs = sh.blocks;
   double* r = reinterpret_cast<double*>(work);
   double* z = r + n;
   double* w = z + n;
@@ -434,9 +459,9 @@ int pcg_run(const char* name, int trust, double radius, int32_t n, const int32_t
       double* p_old = (it & 1) ? pb : pa;
       double* p_new = (it & 1) ? pa : pb;
       if (precision == FNE_FP32)
-        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+        cg_spmv<float>(sh, P, (const float*)vals, z, p_old, p_new, w, x, st, part, s);
       else
-        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+        cg_spmv<double>(sh, P, (const double*)vals, z, p_old, p_new, w, x, st, part, s);
       hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, P, part, nbs, st);
       hipLaunchKernelGGL(cg_update_kernel, dim3(nbv), dim3(kBlock), 0, s, P, p_new, w, x, r, z, st, part);
       hipLaunchKernelGGL(cg_beta_finalize, dim3(1), dim3(1024), 0, s, P, part, nbv, st);
@@ -483,8 +508,8 @@ int pcg_graph_run(const char* name, int trust, double radius, int32_t n, const i
     return FNE_EINVAL;
   }
   if (check_every < 1) check_every = 1;
-  const int lpr = lanes_for(n, nnz64);
-  const int nbs = int(spmv_blocks(n, lpr));
+  const CgSpmvShape sh = cg_spmv_shape(n, nnz64, indices, vals);
+  const int nbs = sh.blocks;
   // r lives in the preconditioner's input buffer, z in its output buffer
   double* r = pc_in;
   const double* z = pc_out;
@@ -512,9 +537,9 @@ int pcg_graph_run(const char* name, int trust, double radius, int32_t n, const i
       double* p_old = (it & 1) ? pb : pa;
       double* p_new = (it & 1) ? pa : pb;
       if (precision == FNE_FP32)
-        cg_spmv<float>(lpr, P, (const float*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+        cg_spmv<float>(sh, P, (const float*)vals, z, p_old, p_new, w, x, st, part, s);
       else
-        cg_spmv<double>(lpr, P, (const double*)vals, z, p_old, p_new, w, x, st, part, nbs, s);
+        cg_spmv<double>(sh, P, (const double*)vals, z, p_old, p_new, w, x, st, part, s);
       hipLaunchKernelGGL(cg_alpha_finalize, dim3(1), dim3(1024), 0, s, P, part, nbs, st);
       hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nbv), dim3(kBlock), 0, s, n, p_new, w, x, r, st);
       FNE_HIP_CHECK(hipGraphLaunch(pc, s));

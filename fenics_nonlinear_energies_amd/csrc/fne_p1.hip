@@ -35,22 +35,61 @@ struct P1Elem {
   double dvx[3], dvy[3], vol;
 };
 
-// element e of SoA arrays with stride ne: codes + geometry, then the gathered nodal values
-__device__ __forceinline__ void p1_load_raw(const int32_t* __restrict__ edof, const double* __restrict__ egeo,
-                                            long ne, long e, int c[3], P1Elem& el) {
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    c[a] = edof[a * ne + e];
-    el.dvx[a] = egeo[a * ne + e];
-    el.dvy[a] = egeo[(3 + a) * ne + e];
-  }
-  el.vol = egeo[6 * ne + e];
+// Geometry of a triangle from its vertex coordinates (FNE_FLAG_NODE_GEO): 16 bytes per vertex,
+// gathered like the nodal values and shared by the vertex's elements in cache, instead of 56
+// bytes per element.  Each operation is rounded on its own (the library is built with
+// -ffp-contract=off; FP64 division is correctly rounded) and in the order of the host's
+// triangle operators, so dvx, dvy and vol equal the host-computed arrays bit for bit.
+__device__ __forceinline__ void p1_node_geometry(const double2* __restrict__ xy, const int c[3], P1Elem& el) {
+  const double2 p0 = xy[c[0]], p1 = xy[c[1]], p2 = xy[c[2]];
+  const double det = (p1.x - p0.x) * (p2.y - p0.y) - (p2.x - p0.x) * (p1.y - p0.y);
+  el.dvx[0] = (p1.y - p2.y) / det;
+  el.dvx[1] = (p2.y - p0.y) / det;
+  el.dvx[2] = (p0.y - p1.y) / det;
+  el.dvy[0] = (p2.x - p1.x) / det;
+  el.dvy[1] = (p0.x - p2.x) / det;
+  el.dvy[2] = (p1.x - p0.x) / det;
+  el.vol = 0.5 * fabs(det);
 }
 
+// element e of SoA arrays with stride ne: codes + geometry (NG: from the vertex coordinates xy)
+template <bool NG>
+__device__ __forceinline__ void p1_load_raw(const int32_t* __restrict__ edof, const double* __restrict__ egeo,
+                                            const double2* __restrict__ xy, long ne, long e, int c[3],
+                                            P1Elem& el) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) c[a] = edof[a * ne + e];
+  if constexpr (NG) {
+    p1_node_geometry(xy, c, el);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      el.dvx[a] = egeo[a * ne + e];
+      el.dvy[a] = egeo[(3 + a) * ne + e];
+    }
+    el.vol = egeo[6 * ne + e];
+  }
+}
+
+__device__ __forceinline__ const double2* p1_xy(const fne_plan_desc& d) {
+  return reinterpret_cast<const double2*>(d.dof_xy);
+}
+
+// the plan's geometry form chosen at run time (uniform over the launch)
+__device__ __forceinline__ void p1_load_raw_any(const fne_plan_desc& d, const int32_t* __restrict__ edof,
+                                                const double* __restrict__ egeo, long ne, long e, int c[3],
+                                                P1Elem& el) {
+  if (d.flags & FNE_FLAG_NODE_GEO)
+    p1_load_raw<true>(edof, egeo, p1_xy(d), ne, e, c, el);
+  else
+    p1_load_raw<false>(edof, egeo, p1_xy(d), ne, e, c, el);
+}
+
+// element e of the plan's element arrays, then the gathered nodal values
 __device__ __forceinline__ void p1_load(const fne_plan_desc& d, int e, const double* __restrict__ x,
                                         const double* __restrict__ dvals, P1Elem& el) {
   int c[3];
-  p1_load_raw(d.edof, d.egeo, d.n_elem, e, c, el);
+  p1_load_raw_any(d, d.edof, d.egeo, d.n_elem, e, c, el);
 #pragma unroll
   for (int a = 0; a < 3; ++a) el.v[a] = gather_dof(c[a], d.n_free, x, dvals);
 }
@@ -259,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void p1_color_kernel(fne_plan_desc d, const
     const int sl = tid + k * B;
     act[k] = sl < ne;
     if (act[k]) {
-      p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code[k], el[k]);
+      p1_load_raw_any(d, d.pedof, d.pgeo, d.n_pe, eb + sl, code[k], el[k]);
       item[k] = d.pitem[eb + sl];
     }
   }
@@ -338,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void p1_color_kernel(fne_plan_desc d, const
 // ---------------------------------------------------------------------------------------------
 constexpr int kStagedPrefetch = 8;
 
-template <int FAM, typename HT, bool WANT_E, bool WANT_G, bool WANT_H>
+template <int FAM, typename HT, bool WANT_E, bool WANT_G, bool WANT_H, bool NG>
 __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, const double* __restrict__ x,
                                                            const double* __restrict__ dvals,
                                                            const double* __restrict__ f,
@@ -377,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, cons
   for (int sl = tid; sl < ne; sl += B) {
     P1Elem el;
     int code[3];
-    p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code, el);
+    p1_load_raw<NG>(d.pedof, d.pgeo, p1_xy(d), d.n_pe, eb + sl, code, el);
 #pragma unroll
     for (int a = 0; a < 3; ++a) el.v[a] = gather_dof(code[a], d.n_free, x, dvals);
     if (WANT_E && sl < nown) acc_add(es, ec, p1_energy<FAM>(el, prm, d.qdata, d.nq));
@@ -504,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void p1_staged_hvp_kernel(fne_plan_desc d, 
   for (int sl = tid; sl < ne; sl += B) {
     P1Elem el;
     int code[3];
-    p1_load_raw(d.pedof, d.pgeo, d.n_pe, eb + sl, code, el);
+    p1_load_raw_any(d, d.pedof, d.pgeo, d.n_pe, eb + sl, code, el);
     double ve[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -635,7 +674,8 @@ int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
     }
     const bool sh = want_h || op == kOpFused;
     const bool sg = want_g || (op == kOpFused && c.g != nullptr);
-    if (!d.pedof || !d.pgeo || !d.pown || !d.inc || !d.inc_ptr) {
+    const bool ng = (d.flags & FNE_FLAG_NODE_GEO) != 0;
+    if (!d.pedof || (!ng && !d.pgeo) || !d.pown || !d.inc || !d.inc_ptr) {
       fne_set_error("staged patch arrays missing from the plan");
       return FNE_EINVAL;
     }
@@ -653,10 +693,16 @@ int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
     }
     const dim3 grid(d.n_patch), block(kBlock);
     double2* part = plan->partials;
-#define ST_GO(HT, E, G, H)                                                                              \
-  hipLaunchKernelGGL((p1_staged_kernel<FAM, HT, E, G, H>), grid, block, shm, c.stream, d, c.x, c.dvals, c.f, c.g, \
-                     (HT*)c.vals, part)
-#define ST_MR(HT, E, G, H) ST_GO(HT, E, G, H)
+#define ST_GO(HT, E, G, H, NG)                                                                          \
+  hipLaunchKernelGGL((p1_staged_kernel<FAM, HT, E, G, H, NG>), grid, block, shm, c.stream, d, c.x, c.dvals, c.f, \
+                     c.g, (HT*)c.vals, part)
+#define ST_MR(HT, E, G, H) \
+  do {                     \
+    if (ng)                \
+      ST_GO(HT, E, G, H, true); \
+    else                   \
+      ST_GO(HT, E, G, H, false); \
+  } while (0)
     if (!sh) {
       ST_MR(double, false, true, false);
     } else if (c.precision == FNE_FP32) {
@@ -686,7 +732,7 @@ int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   }
   const bool want_hh = want_h || op == kOpFused;
   const bool want_gg = want_g || (op == kOpFused && c.g != nullptr);
-  if (!d.pedof || !d.pgeo || !d.pitem || !d.pcolors) {
+  if (!d.pedof || (!(d.flags & FNE_FLAG_NODE_GEO) && !d.pgeo) || !d.pitem || !d.pcolors) {
     fne_set_error("colored patch arrays missing from the plan");
     return FNE_EINVAL;
   }
@@ -738,7 +784,8 @@ template <int FAM>
 int launch_p1_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y,
                   hipStream_t stream) {
   const fne_plan_desc& d = plan->d;
-  if (!(d.flags & FNE_FLAG_STAGED) || !d.pedof || !d.pgeo || !d.inc || !d.inc_ptr) {
+  const bool ng = (d.flags & FNE_FLAG_NODE_GEO) != 0;
+  if (!(d.flags & FNE_FLAG_STAGED) || !d.pedof || (!ng && !d.pgeo) || !d.inc || !d.inc_ptr) {
     fne_set_error("P1 Hessian-vector products need a staged patch plan (FNE_FLAG_STAGED)");
     return FNE_EUNSUPPORTED;
   }

@@ -177,8 +177,11 @@ def slope_predictor(p, asm, net, mode, lam_scale=1.0):
 
 def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.0, eval_h=0.5, degree=4,
                           ksp_max_it=200, ksp_rtol=1e-3, pc_refresh_its=None, mixed="fp32pc", hidden=128, layers=4,
-                          epochs=60, lr=2e-3, seed=0, guard=True, with_reference=True, newton_kwargs=None, log=print):
+                          epochs=60, lr=2e-3, seed=0, guard=True, with_reference=True, newton_kwargs=None,
+                          pc_type="mg", log=print):
     """Strength-reduction continuation on the slope: previous (FP64) / extrapolated / learned (mixed).
+
+    ``pc_type``: "mg" (the same-mesh p-multigrid of the reference's P4 runs) or "gamg".
 
     ``mixed="fp32pc"``: FP64 Hessian and Krylov operator, multigrid stored in FP32 (the P4
     nu = 0.49 Hessian is too ill-conditioned for FP32 storage at ~1.5M DOFs: rounding its
@@ -197,7 +200,7 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
     feats, targs = [], []
     pred = None
     for mat in train_materials:
-        p, asm = _slope_assembler(train_h, degree, mat, mix_h, ksp_max_it, pc_precision=mix_pc)
+        p, asm = _slope_assembler(train_h, degree, mat, mix_h, ksp_max_it, pc_type=pc_type, pc_precision=mix_pc)
         x, _ = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)
         pred = slope_predictor(p, asm, None, "extrapolate")
         holder = {}
@@ -213,7 +216,8 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
     stats = (pred.feat_mean, pred.feat_std, pred.targ_std)
     losses = net.train_mse(X, T, epochs=epochs, batch=16384, lr=lr, lr_final=lr * 0.05, seed=seed)
     log(f"trained on {X.shape[0]} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
-    out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "variants": {}}
+    out = {"train_samples": int(X.shape[0]), "loss_first": losses[0], "loss_last": losses[-1], "pc_type": pc_type,
+           "variants": {}}
     # (name, initial guess, Hessian precision, PC precision, keep the PC, rebuild above, KSP cap)
     variants = [("previous_fp64", "previous", "fp64", None, True, pc_refresh_its, ksp_max_it),
                 (f"extrapolate_{mixed}", "extrapolate", mix_h, mix_pc, True, pc_refresh_its, ksp_max_it),
@@ -223,7 +227,7 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         # previous state, FP64, CG capped at 200 iterations, PC set up at every Newton iteration
         variants.insert(0, ("reference_fp64", "previous", "fp64", None, False, None, 200))
     for name, mode, prec, pc_prec, reuse, refresh, max_it in variants:
-        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, max_it, pc_precision=pc_prec)
+        p, asm = _slope_assembler(eval_h, degree, eval_material, prec, max_it, pc_type=pc_type, pc_precision=pc_prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, el_its = elastic_initial_guess(asm, rtol=1e-8, max_it=20 * ksp_max_it)

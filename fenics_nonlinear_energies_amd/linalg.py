@@ -42,6 +42,28 @@ def lanes_per_row(avg_row_len: float) -> int:
     return lpr
 
 
+# precision-argument field (bits 16..19) of fne_spmv / fne_vcycle_op / fne_vcycle_cheb: the
+# CSR-stream form (a workgroup streams 256 / lanes consecutive rows' entries through LDS with
+# 16-byte loads, then lanes per row sum them)
+SPMV_FORM_STREAM = 1 << 16
+
+
+def stream_lanes_per_row(avg_row_len: float) -> int:
+    """Lanes per row of the CSR-stream form (``csr_stream_lanes`` of the library): about eight
+    entries per lane -- 1 for the 7-entry P1 rows (256 rows per workgroup), 4 for the 40-50-entry
+    vector rows."""
+    lpr = 1
+    while lpr < 64 and 2 * lpr <= avg_row_len / 8.0:
+        lpr *= 2
+    return lpr
+
+
+def spmv_code(fne_precision: int, avg_row_len: float) -> int:
+    """The precision argument of the CSR row kernels: value type, CSR-stream form and its lanes
+    per row (kernels given unaligned arrays fall back to lanes per row with the same count)."""
+    return int(fne_precision) | (stream_lanes_per_row(avg_row_len) << 8) | SPMV_FORM_STREAM
+
+
 class DeviceCSR:
     """Square CSR matrix on the GPU with a fixed pattern; values FP64 or FP32."""
 
@@ -52,8 +74,8 @@ class DeviceCSR:
         self.nnz = int(self.indices.numel())
         self.set_precision(precision)
         row_len = np.diff(np.asarray(indptr, dtype=np.int64))
-        avg = float(row_len.mean()) if row_len.size else 1.0
-        self.lanes_per_row = lanes_per_row(avg)
+        self.avg_row_len = float(row_len.mean()) if row_len.size else 1.0
+        self.lanes_per_row = stream_lanes_per_row(self.avg_row_len)
 
     def set_precision(self, precision: str) -> None:
         if precision not in ("fp64", "fp32"):
@@ -76,7 +98,7 @@ class DeviceCSR:
         self.mult_tensor(x.t, y.t)
 
     def mult_tensor(self, x: torch.Tensor, y: torch.Tensor) -> None:
-        prec = self.fne_precision | (self.lanes_per_row << 8)
+        prec = spmv_code(self.fne_precision, self.avg_row_len)
         native.check(
             native.lib().fne_spmv(
                 self.n,

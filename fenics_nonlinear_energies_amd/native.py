@@ -28,6 +28,7 @@ FNE_FLAG_ABS_DET = 1
 FNE_FLAG_ROW_GATHER = 2
 FNE_FLAG_COLORED = 4
 FNE_FLAG_STAGED = 8
+FNE_FLAG_NODE_GEO = 16
 FNE_PCG_NOALLOC_MAX_BYTES = 16 << 20
 
 _i32p = C.POINTER(C.c_int32)
@@ -71,6 +72,7 @@ class FnePlanDesc(C.Structure):
         ("qdata", C.c_void_p),
         ("nq", C.c_int32),
         ("flags", C.c_int32),
+        ("dof_xy", C.c_void_p),
     ]
 
 

@@ -82,6 +82,9 @@ class GLReorderedElementAssembler(ReorderedElementAssemblerBase):
             axis=0,
         )
 
+    def _geometry_from_nodes(self) -> np.ndarray | None:
+        return mesh2d.triangle_geometry_rows(self.params["nodes"], self.params["elems"])
+
     def _kernel_params(self) -> list[float]:
         return [float(self.params["eps"])]
 

@@ -45,6 +45,20 @@ def triangle_operators(nodes: np.ndarray, elems: np.ndarray):
     return dvx, dvy, vol
 
 
+def triangle_geometry_rows(nodes: np.ndarray, elems: np.ndarray) -> np.ndarray | None:
+    """(7, n_elem) rows dvx0..2, dvy0..2, vol from the vertex coordinates -- the operation order
+    the P1 kernels use with FNE_FLAG_NODE_GEO; None when the mesh has a degenerate triangle or
+    is not planar."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    if nodes.ndim != 2 or nodes.shape[1] != 2 or np.asarray(elems).shape[-1] != 3:
+        return None
+    try:
+        dvx, dvy, vol = triangle_operators(nodes, elems)
+    except ValueError:
+        return None
+    return np.concatenate([dvx.T, dvy.T, vol[None, :]], axis=0)
+
+
 def rectangle_boundary_mask(nodes: np.ndarray, x0, x1, y0, y1, tol: float = 1e-12) -> np.ndarray:
     x = nodes[:, 0]
     y = nodes[:, 1]

@@ -63,6 +63,9 @@ class PLaplaceReorderedElementAssembler(ReorderedElementAssemblerBase):
             axis=0,
         )
 
+    def _geometry_from_nodes(self) -> np.ndarray | None:
+        return mesh2d.triangle_geometry_rows(self.params["nodes"], self.params["elems"])
+
     def _kernel_params(self) -> list[float]:
         return [float(self.params["p"])]
 

@@ -92,6 +92,9 @@ typedef struct fne_plan_desc {
   const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]) */
   int32_t nq;             /* quadrature points */
   int32_t flags;          /* FNE_FLAG_* */
+  /* P1 with FNE_FLAG_NODE_GEO: vertex coordinates (x, y) of every DOF code, [n_free + n_total][2]
+     (code c < n_free: free DOF c; else Dirichlet entry c - n_free); egeo/pgeo are not read */
+  const double* dof_xy;
 } fne_plan_desc;
 
 #define FNE_FLAG_ABS_DET 1 /* Neo-Hookean: |det F| (serial path hyperelasticity/jax/jax_energy.py:26) */
@@ -100,6 +103,11 @@ typedef struct fne_plan_desc {
 #define FNE_FLAG_STAGED 8 /* P1: staged patch assembly.  Elements of a patch are evaluated into LDS,
                               then one thread per row sums its incidence list (inc_ptr/inc: slot[0:13)
                               | a[13:15) | pos_b[15+5b:20+5b), 31 = none) in registers */
+#define FNE_FLAG_NODE_GEO 16 /* P1: element geometry (dvx, dvy, vol) evaluated in the kernels from the
+                                vertex coordinates dof_xy with the host's operation order
+                                (det = (x1-x0)(y2-y0) - (x2-x0)(y1-y0), dvx = (y1-y2, y2-y0, y0-y1)/det,
+                                dvy = (x2-x1, x0-x2, x1-x0)/det, vol = |det|/2), in place of reading
+                                n_geo = 7 doubles per element */
 #define FNE_FLAG_ROW_GATHER 2 /* P1: one thread per CSR row gathers its incident elements (no LDS
                                  staging); inc holds uint64 items e[0:32) a[32:34) pos_b[40+8b:48+8b) */
 

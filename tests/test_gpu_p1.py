@@ -135,3 +135,44 @@ def test_fused_energy_grad_hessian_matches_separate_calls(require_gpu, amode):
     assert torch.equal(h_sep, asm.A.data)
     ref = assembly.assemble("gl", gp, u)
     assert abs(e_fused - ref["e"]) <= 1e-13 * abs(ref["e"])
+
+
+@pytest.mark.parametrize("family", ["plaplace", "gl"])
+@pytest.mark.parametrize("amode", ["patch", "staged", "rowgather"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_node_geometry_bitwise_equal_to_geometry_arrays(require_gpu, family, amode, prec):
+    """FNE_FLAG_NODE_GEO (element geometry from the vertex coordinates in the kernels) gives
+    the same bits as reading the geometry arrays: energy, gradient, CSR values, the fused pass,
+    element outputs and (staged plans) Hessian-vector products, on the reference's level-5 mesh."""
+    d, adj = fixtures.plaplace(5) if family == "plaplace" else fixtures.gl(5)
+    cls = PLaplaceReorderedElementAssembler if family == "plaplace" else GLReorderedElementAssembler
+    a_arr = cls(d, None, adj, assembly_mode=amode, hessian_precision=prec, element_geometry="arrays")
+    a_nod = cls(d, None, adj, assembly_mode=amode, hessian_precision=prec, element_geometry="nodes")
+    assert not a_arr.node_geometry and a_nod.node_geometry
+    assert "egeo" not in a_nod._dev and "pgeo" not in a_nod._dev
+    rng = np.random.default_rng(21)
+    u = rng.random(d["freedofs"].size) if family == "plaplace" else rng.uniform(-1, 1, d["freedofs"].size)
+    xa = a_arr.create_vec(u[a_arr.layout.perm])
+    xn = a_nod.create_vec(u[a_nod.layout.perm])
+    assert torch.equal(xa.t, xn.t)
+    assert a_arr.energy_fn(xa) == a_nod.energy_fn(xn)
+    ga, gn = xa.duplicate(), xn.duplicate()
+    a_arr.gradient_fn(xa, ga)
+    a_nod.gradient_fn(xn, gn)
+    assert torch.equal(ga.t, gn.t)
+    a_arr.assemble_hessian(xa)
+    a_nod.assemble_hessian(xn)
+    assert torch.equal(a_arr.A.data, a_nod.A.data)
+    ea = float(a_arr.energy_grad_hessian_device(xa.t, ga.t).item())
+    en = float(a_nod.energy_grad_hessian_device(xn.t, gn.t).item())
+    assert ea == en and torch.equal(ga.t, gn.t) and torch.equal(a_arr.A.data, a_nod.A.data)
+    for ta, tn in zip(a_arr.element_eval(xa), a_nod.element_eval(xn)):
+        assert torch.equal(ta, tn)
+    if amode == "staged":
+        v = torch.as_tensor(rng.standard_normal(xa.t.numel()), device=xa.t.device)
+        ya, yn = torch.empty_like(v), torch.empty_like(v)
+        a_arr.hvp_device(xa.t, v, ya)
+        a_nod.hvp_device(xn.t, v, yn)
+        assert torch.equal(ya, yn)
+    a_arr.cleanup()
+    a_nod.cleanup()
