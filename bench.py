@@ -335,7 +335,7 @@ def main():
     achieved = b_asm / (kern_ms * 1e-3) / 1e9
     traffic = None
     # HBM bytes per launch of the assembly kernel from the committed FETCH_SIZE/WRITE_SIZE summary
-    geo_tag = "" if asm.node_geometry else "_geoarrays"
+    geo_tag = "_nodegeo" if asm.node_geometry else ""
     tfile = ROOT / "profiles" / f"p1_staged_gl{args.n}_{args.precision}{geo_tag}_traffic.md"
     if tfile.exists():
         for line in tfile.read_text().splitlines():
