@@ -181,6 +181,14 @@ _SIGNATURES = [
          C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p, C.c_int64,
          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p],
     ),
+    ("fne_gmres_workspace_bytes", C.c_int64, [C.c_int32, C.c_int32, C.c_int32]),
+    (
+        "fne_gmres",
+        C.c_int,
+        [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+         C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p],
+    ),
     ("fne_mlp_forward", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("fne_mlp_workspace_bytes", C.c_int64, [C.c_void_p, C.c_int64]),
     (
@@ -244,7 +252,16 @@ def ptr(t) -> int | None:
 
 
 def stream_ptr(stream=None) -> int | None:
+    """Raw HIP stream of ``stream`` or of torch's current stream on the current device (None
+    for the null stream).  The current stream is read through torch's C accessors: the public
+    ``torch.cuda.current_stream()`` costs about 8 us per call in Python, which adds up over the
+    tens of thousands of launches of a Newton load path."""
     import torch
 
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return int(s.cuda_stream) or None
+    if stream is not None:
+        return int(stream.cuda_stream) or None
+    try:
+        raw = torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
+    except AttributeError:
+        raw = torch.cuda.current_stream().cuda_stream
+    return int(raw) or None

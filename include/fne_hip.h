@@ -315,6 +315,27 @@ int fne_pstcg_graph(int32_t n, const int32_t* indptr, const int32_t* indices, co
                     fne_stream_t stream);
 
 /*
+ * Replaces ksp.solve with ksp_type="gmres" (left preconditioning; the reference's
+ * Ginzburg-Landau runs, ginzburg_landau/jax_petsc/solver.py:31-32, through
+ * scalar_problem_driver.py:174-196) and ksp_type="fgmres" (flexible = 1: right preconditioning
+ * with the preconditioned basis kept; the slope p-multigrid runs, slope_stability/jax_petsc/
+ * solver.py:1275).  PETSc KSPGMRES / KSPFGMRES semantics: zero initial guess, restart length
+ * `restart` (1..512), classical Gram-Schmidt, Givens rotations; GMRES tests the preconditioned
+ * residual estimate against max(rtol ||B b||, atol), FGMRES the unpreconditioned one; dtol
+ * divergence, happy breakdown.  Preconditioner: pc_graph_exec (a recorded graph mapping pc_in
+ * to pc_out, as fne_pcg_graph) when given, else inv_diag (Jacobi), else none.  The host reads
+ * the device state every check_every Arnoldi steps (0: only at the end of a cycle).  Workspace:
+ * fne_gmres_workspace_bytes(n, restart, flexible) bytes of device memory.  Reasons: 2 rtol,
+ * 3 atol, -3 iteration cap, -4 dtol, -9 NaN/inf.
+ */
+int64_t fne_gmres_workspace_bytes(int32_t n, int32_t restart, int32_t flexible);
+int fne_gmres(int32_t n, const int32_t* indptr, const int32_t* indices, const void* vals, int32_t precision,
+              const double* inv_diag, void* pc_graph_exec, double* pc_in, const double* pc_out, const double* b,
+              double* x, double rtol, double atol, double dtol, int32_t max_it, int32_t restart, int32_t flexible,
+              int32_t check_every, void* work, int64_t work_bytes, int32_t* its_out, int32_t* reason_out,
+              double* rnorm_out, fne_stream_t stream);
+
+/*
  * fne_pcg without a workspace argument, for callers that cannot allocate device memory (code
  * that must not call hipMalloc, such as tight solve loops or harnesses that forbid it).  No
  * counterpart in the reference.  Same arguments, semantics and outputs as fne_pcg minus
