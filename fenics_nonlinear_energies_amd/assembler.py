@@ -211,7 +211,8 @@ class ReorderedElementAssemblerBase:
         if kind not in ("richardson", "jacobi", "chebyshev"):
             raise ValueError(f"unsupported mg_levels_ksp_type {kind!r} (richardson | chebyshev)")
         self.ksp.mg_smoother = "chebyshev" if kind == "chebyshev" else "jacobi"
-        if "mg_levels_ksp_max_it" in opts:
+        self._pc_sweeps_given = "mg_levels_ksp_max_it" in opts
+        if self._pc_sweeps_given:
             self.ksp.mg_sweeps = int(opts["mg_levels_ksp_max_it"])
 
     def _configure_ksp(self, ksp) -> None:

@@ -172,7 +172,7 @@ class NodalPredictor:
     """
 
     def __init__(self, free_node_dofs: np.ndarray, coords: np.ndarray, *, state_offset, disp_scale: float,
-                 load_scale: float, net: NodeMLP | None = None, mode: str = "learned"):
+                 load_scale: float, net: NodeMLP | None = None, mode: str = "learned", first_step: bool = False):
         self.idx = torch.as_tensor(np.asarray(free_node_dofs, dtype=np.int64))  # (n_nodes_free, dim)
         self.coords = np.asarray(coords, dtype=np.float64)
         self.dim = self.idx.shape[1]
@@ -181,6 +181,10 @@ class NodalPredictor:
         self.load_scale = float(load_scale)
         self.net = net
         self.mode = str(mode)
+        # first_step: the network also predicts the first step of a path, from the start state
+        # alone (features with u_{k-1} = u_k, i.e. no velocity); trained on the start sample of
+        # every history (training_pairs)
+        self.first_step = bool(first_step)
         self._dev = None
         self.feat_mean = self.feat_std = self.targ_std = None
         self.energy_fn = None  # optional: accept the learned guess only if its energy is lower
@@ -227,9 +231,14 @@ class NodalPredictor:
         return ((u_next - pred)[idx].float() / self.disp_scale).contiguous()
 
     def __call__(self, history, load_next):
-        if self.mode == "previous" or len(history) < 2:
+        if self.mode == "previous" or not history:
             return None  # caller keeps the previous converged state
-        u_k, u_km1 = history[-1][1], history[-2][1]
+        if len(history) < 2:
+            if not (self.first_step and self.mode == "learned" and self.net is not None):
+                return None
+            u_k = u_km1 = history[-1][1]  # the start state: the guess is the previous state
+        else:
+            u_k, u_km1 = history[-1][1], history[-2][1]
         guess = 2.0 * u_k - u_km1
         if self.mode == "extrapolate" or self.net is None:
             return guess
@@ -250,10 +259,16 @@ class NodalPredictor:
 
 def training_pairs(predictor: NodalPredictor, history) -> tuple[torch.Tensor, torch.Tensor]:
     """(raw features, raw targets) of every (k-1, k, k+1) triple of consecutive, equally spaced
-    loads in a converged history.  The history holds converged steps only, so a step that did
+    loads in a converged history (and, for a predictor with ``first_step``, the start sample
+    (0, 0, 1) with u_{-1} = u_0).  The history holds converged steps only, so a step that did
     not converge leaves a gap; a triple across it would pair a two-step increment with the
     one-step extrapolation the features describe, and is skipped."""
     xs, ts = [], []
+    if predictor.first_step and len(history) >= 2:
+        # start sample: (u_0, u_0) -> u_1, the first step from the start state alone
+        (_, u_0), (load_1, u_1) = history[0], history[1]
+        xs.append(predictor.raw_features(u_0, u_0, load_1))
+        ts.append(predictor.target(u_0, u_0, u_1))
     for j in range(1, len(history) - 1):
         (load_km1, u_km1), (load_k, u_k), (load_next, u_next) = history[j - 1], history[j], history[j + 1]
         h0, h1 = float(load_k) - float(load_km1), float(load_next) - float(load_k)

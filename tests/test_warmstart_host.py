@@ -115,3 +115,29 @@ def test_training_pairs_skip_triples_across_a_missing_step():
     assert torch.allclose(x[:, -1], torch.full((n,), 2.5 / 2.0))
     with pytest.raises(ValueError):
         training_pairs(pred, gapped[:4])
+
+
+def test_first_step_prediction_and_start_sample():
+    """With first_step the network predicts step 1 from the start state alone (features with
+    u_{k-1} = u_k, correction on top of the start state), and training_pairs adds that start
+    sample; without it the first step keeps the previous state and no start sample is made."""
+    pred, history, idx = _predictor("learned", net=_ConstNet(0.25))
+    pred.targ_std = torch.tensor([2.0, 2.0, 2.0])
+    assert pred(history[:1], 0.5) is None  # off by default
+    pred.first_step = True
+    u0 = history[0][1]
+    g = pred(history[:1], 0.5)
+    mask = torch.zeros_like(u0, dtype=torch.bool)
+    mask[idx.reshape(-1)] = True
+    assert torch.allclose(g[mask], u0[mask] + 0.5 * 0.25 * 2.0, rtol=0, atol=1e-12)
+    assert torch.equal(g[~mask], u0[~mask])
+    n = idx.shape[0]
+    x, t = training_pairs(pred, history)
+    assert x.shape[0] == 3 * n  # start sample + the two triples
+    d = pred.dim
+    assert torch.equal(x[:n, d:2 * d], x[:n, 2 * d:3 * d])  # u_{k-1} = u_k in the start sample
+    assert torch.allclose(x[:n, -1], torch.full((n,), history[1][0] / 2.0))
+    assert torch.allclose(t[:n].double(), (history[1][1] - u0)[idx] / 0.5, rtol=1e-6, atol=1e-6)
+    ext = _predictor("extrapolate")[0]
+    ext.first_step = True
+    assert ext(history[:1], 0.5) is None  # extrapolation needs two states
