@@ -25,9 +25,17 @@ constexpr int kFamPL = FNE_KIND_PLAPLACE;
 constexpr int kFamGL = FNE_KIND_GINZBURG_LANDAU;
 constexpr int kFamPL3 = 101;
 constexpr int kFamPL2 = 102;
+// Ginzburg-Landau with the 3-point rule of every GL mesh (nq == 3 at compile time): one pass for
+// energy, gradient and Hessian with explicit FMAs and the products w_q ip[a][q] ip[b][q] read
+// from the quadrature table
+constexpr int kFamGL3 = 103;
 template <int fam>
 constexpr bool is_plaplace() {
   return fam == kFamPL || fam == kFamPL3 || fam == kFamPL2;
+}
+template <int fam>
+constexpr bool is_gl() {
+  return fam == kFamGL || fam == kFamGL3;
 }
 
 struct P1Elem {
@@ -101,6 +109,46 @@ __device__ __forceinline__ double pl_pow_half(double s, double p) {
   return pow(s, 0.5 * p);
 }
 
+// Ginzburg-Landau, 3 quadrature points: energy E, gradient ge and Hessian he (sym3 order) in
+// one pass.  Table layout (10 nq values): ip[a][q] at 3a+q, w[q] at 9+q, then the products
+// W[k][q] = w_q ip[a_k][q] ip[b_k][q] at 12+3k+q for k = 00 11 22 01 02 12.  The same formulas
+// as p1_energy / p1_grad_hess, evaluated with fused multiply-adds:
+//   u_q = sum_a v_a ip[a][q],  t_q = u_q^2 - 1
+//   E = vol (eps/2 s + 1/4 sum_q w_q t_q^2)
+//   ge_a = vol (eps t_a + sum_q (w_q t_q u_q) ip[a][q])
+//   he_k = vol (eps K_k + sum_q (3 u_q^2 - 1) W[k][q])
+__device__ __forceinline__ void gl3_eval(const P1Elem& el, double eps, const double* __restrict__ qd, double& E,
+                                         double ge[3], double he[6]) {
+  const double gx = fma(el.v[2], el.dvx[2], fma(el.v[1], el.dvx[1], el.v[0] * el.dvx[0]));
+  const double gy = fma(el.v[2], el.dvy[2], fma(el.v[1], el.dvy[1], el.v[0] * el.dvy[0]));
+  const double s = fma(gy, gy, gx * gx);
+  double pot = 0.0, gq[3] = {0.0, 0.0, 0.0}, hq[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double uq = fma(el.v[2], qd[6 + q], fma(el.v[1], qd[3 + q], el.v[0] * qd[q]));
+    const double u2 = uq * uq;
+    const double tq = u2 - 1.0;
+    const double wt = qd[9 + q] * tq;
+    pot = fma(wt, tq, pot);
+    const double gc = wt * uq;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gq[a] = fma(gc, qd[3 * a + q], gq[a]);
+    const double hc = fma(3.0, u2, -1.0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) hq[k] = fma(hc, qd[12 + 3 * k + q], hq[k]);
+  }
+  E = el.vol * fma(0.25, pot, (0.5 * eps) * s);
+  const int ia[6] = {0, 1, 2, 0, 0, 1};
+  const int ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ge[a] = el.vol * fma(eps, fma(gy, el.dvy[a], gx * el.dvx[a]), gq[a]);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = ia[k], b = ib[k];
+    he[k] = el.vol * fma(eps, fma(el.dvy[a], el.dvy[b], el.dvx[a] * el.dvx[b]), hq[k]);
+  }
+}
+
 template <int fam>
 __device__ __forceinline__ double p1_energy(const P1Elem& el, const double* prm, const double* __restrict__ qd,
                                             int nq) {
@@ -111,6 +159,11 @@ __device__ __forceinline__ double p1_energy(const P1Elem& el, const double* prm,
     gy += el.v[a] * el.dvy[a];
   }
   const double s = gx * gx + gy * gy;
+  if constexpr (fam == kFamGL3) {
+    double E, ge[3], he[6];
+    gl3_eval(el, prm[0], qd, E, ge, he);
+    return E;
+  }
   if constexpr (fam == kFamPL3) {
     return (1.0 / 3.0) * (s * sqrt(s)) * el.vol;
   } else if constexpr (fam == kFamPL2) {
@@ -133,6 +186,11 @@ __device__ __forceinline__ double p1_energy(const P1Elem& el, const double* prm,
 template <int fam>
 __device__ __forceinline__ void p1_grad_hess(const P1Elem& el, const double* prm, const double* __restrict__ qd,
                                              int nq, bool want_g, bool want_h, double ge[3], double he[6]) {
+  if constexpr (fam == kFamGL3) {
+    double E;
+    gl3_eval(el, prm[0], qd, E, ge, he);
+    return;
+  }
   double gx = 0.0, gy = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -208,6 +266,19 @@ __device__ __forceinline__ void p1_grad_hess(const P1Elem& el, const double* prm
       const double K = el.dvx[a] * el.dvx[b] + el.dvy[a] * el.dvy[b];
       he[k] = el.vol * (eps * K + hq[k]);
     }
+  }
+}
+
+// energy, gradient and Hessian of one element for the given family (GL3 in one pass)
+template <int fam>
+__device__ __forceinline__ void p1_eval(const P1Elem& el, const double* prm, const double* __restrict__ qd, int nq,
+                                        bool want_e, bool want_g, bool want_h, double& E, double ge[3],
+                                        double he[6]) {
+  if constexpr (fam == kFamGL3) {
+    gl3_eval(el, prm[0], qd, E, ge, he);
+  } else {
+    if (want_e) E = p1_energy<fam>(el, prm, qd, nq);
+    if (want_g || want_h) p1_grad_hess<fam>(el, prm, qd, nq, want_g, want_h, ge, he);
   }
 }
 
@@ -288,6 +359,8 @@ __global__ __launch_bounds__(kBlock) void p1_color_kernel(fne_plan_desc d, const
     for (int i = tid; i < nrows; i += B) sG[i] = 0.0;
 
   double es = 0.0, ec = 0.0;
+  if (WANT_E && f)
+    for (int i = tid; i < nrows; i += B) acc_add(es, ec, -(f[r0 + i] * x[r0 + i]));
   const double* prm = d.param;
   P1Elem el[EPT];
   int code[EPT][3];
@@ -373,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void p1_color_kernel(fne_plan_desc d, const
 // columns of one incidence are distinct, so their read-modify-writes into the row's part of an
 // LDS copy of the CSR slice are issued together.  Every slice word is touched by one thread in a
 // fixed order (bitwise reproducible, no atomics); the slice then leaves as coalesced stores.
+// The fused energy is the owned elements' energies plus the load term -f.x of the patch rows.
 // HT = float accumulates in FP32 (FP32 Hessian mode), HT = double in FP64.
 // ---------------------------------------------------------------------------------------------
 constexpr int kStagedPrefetch = 8;
@@ -411,6 +485,7 @@ __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, cons
     for (int k = tid; k < nslice; k += B) sVal[k] = HT(0);
 
   double es = 0.0, ec = 0.0;
+  if (WANT_E && f && has_row) acc_add(es, ec, -(f[r] * x[r]));
   const double* prm = d.param;
 #pragma unroll 1
   for (int sl = tid; sl < ne; sl += B) {
@@ -419,10 +494,10 @@ __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, cons
     p1_load_raw<NG>(d.pedof, d.pgeo, p1_xy(d), d.n_pe, eb + sl, code, el);
 #pragma unroll
     for (int a = 0; a < 3; ++a) el.v[a] = gather_dof(code[a], d.n_free, x, dvals);
-    if (WANT_E && sl < nown) acc_add(es, ec, p1_energy<FAM>(el, prm, d.qdata, d.nq));
+    double E = 0.0, ge[3], he[6];
+    p1_eval<FAM>(el, prm, d.qdata, d.nq, WANT_E && sl < nown, WANT_G, WANT_H, E, ge, he);
+    if (WANT_E && sl < nown) acc_add(es, ec, E);
     if (WANT_G || WANT_H) {
-      double ge[3], he[6];
-      p1_grad_hess<FAM>(el, prm, d.qdata, d.nq, WANT_G, WANT_H, ge, he);
       if (WANT_H) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) sH[q * cap + sl] = HT(he[q]);
@@ -598,7 +673,7 @@ __global__ __launch_bounds__(kBlock) void p1_element_kernel(fne_plan_desc d, con
 template <int FAM>
 int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
-  if (FAM == kFamGL && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
+  if (is_gl<FAM>() && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
     fne_set_error("Ginzburg-Landau plan needs 1..16 quadrature points in qdata");
     return FNE_EINVAL;
   }
@@ -780,6 +855,12 @@ int launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
   return FNE_OK;
 }
 
+// Ginzburg-Landau plans with 3 quadrature points whose table carries the w ip ip products
+// (FNE_FLAG_GL_PRODUCTS) take the one-pass GL3 kernels
+bool gl3_table(const fne_plan_desc& d) {
+  return d.nq == 3 && d.qdata != nullptr && (d.flags & FNE_FLAG_GL_PRODUCTS);
+}
+
 template <int FAM>
 int launch_p1_hvp(fne_plan* plan, const double* x, const double* dvals, const double* v, double* y,
                   hipStream_t stream) {
@@ -789,7 +870,7 @@ int launch_p1_hvp(fne_plan* plan, const double* x, const double* dvals, const do
     fne_set_error("P1 Hessian-vector products need a staged patch plan (FNE_FLAG_STAGED)");
     return FNE_EUNSUPPORTED;
   }
-  if (FAM == kFamGL && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
+  if (is_gl<FAM>() && (d.nq <= 0 || d.nq > kMaxQ || d.qdata == nullptr)) {
     fne_set_error("Ginzburg-Landau plan needs 1..16 quadrature points in qdata");
     return FNE_EINVAL;
   }
@@ -811,6 +892,7 @@ int fne_hvp_p1(fne_plan* plan, const double* x, const double* dvals, const doubl
     if (p == 2.0) return launch_p1_hvp<kFamPL2>(plan, x, dvals, v, y, s);
     return launch_p1_hvp<kFamPL>(plan, x, dvals, v, y, s);
   }
+  if (gl3_table(plan->d)) return launch_p1_hvp<kFamGL3>(plan, x, dvals, v, y, s);
   return launch_p1_hvp<kFamGL>(plan, x, dvals, v, y, s);
 }
 
@@ -821,5 +903,6 @@ int fne_launch_p1(fne_plan* plan, FneOp op, const FneCall& c) {
     if (p == 2.0) return launch_p1<kFamPL2>(plan, op, c);
     return launch_p1<kFamPL>(plan, op, c);
   }
+  if (gl3_table(plan->d)) return launch_p1<kFamGL3>(plan, op, c);
   return launch_p1<kFamGL>(plan, op, c);
 }

@@ -29,6 +29,7 @@ FNE_FLAG_ROW_GATHER = 2
 FNE_FLAG_COLORED = 4
 FNE_FLAG_STAGED = 8
 FNE_FLAG_NODE_GEO = 16
+FNE_FLAG_GL_PRODUCTS = 32
 FNE_PCG_NOALLOC_MAX_BYTES = 16 << 20
 
 _i32p = C.POINTER(C.c_int32)

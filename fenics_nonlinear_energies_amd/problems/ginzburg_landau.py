@@ -14,6 +14,7 @@ from __future__ import annotations
 import numpy as np
 
 from ..assembler import ReorderedElementAssemblerBase
+from .. import native
 from ..native import FNE_KIND_GINZBURG_LANDAU
 from . import mesh2d
 
@@ -89,9 +90,16 @@ class GLReorderedElementAssembler(ReorderedElementAssemblerBase):
         return [float(self.params["eps"])]
 
     def _quadrature_table(self) -> np.ndarray | None:
+        """ip[3][nq], w[nq], then the products w_q ip[a][q] ip[b][q] for ab = 00 11 22 01 02 12
+        (FNE_FLAG_GL_PRODUCTS): the Hessian's quadrature weights, one per entry and point."""
         ip = np.asarray(self.params["ip"], dtype=np.float64)
         w = np.asarray(self.params["w"], dtype=np.float64)
-        return np.concatenate([ip.reshape(-1), w])
+        pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+        prod = np.stack([w * ip[a] * ip[b] for a, b in pairs])
+        return np.concatenate([ip.reshape(-1), w, prod.reshape(-1)])
+
+    def _plan_flags(self) -> int:
+        return super()._plan_flags() | native.FNE_FLAG_GL_PRODUCTS
 
     def _build_rhs_owned(self) -> np.ndarray:
         return np.zeros(self.layout.n_free, dtype=np.float64)

@@ -89,7 +89,9 @@ typedef struct fne_plan_desc {
   const int32_t* pcolors;    /* [n_patch] number of colors of each patch (FNE_FLAG_COLORED) */
   /* problem parameters */
   double param[8];        /* p | eps | C1,D1 | E,nu,phi_deg,cohesion,reg */
-  const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]) */
+  const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]; with
+                             FNE_FLAG_GL_PRODUCTS then w[q] ip[a][q] ip[b][q] for ab = 00 11 22 01 02 12,
+                             [6][nq]) */
   int32_t nq;             /* quadrature points */
   int32_t flags;          /* FNE_FLAG_* */
   /* P1 with FNE_FLAG_NODE_GEO: vertex coordinates (x, y) of every DOF code, [n_free + n_total][2]
@@ -110,6 +112,9 @@ typedef struct fne_plan_desc {
                                 n_geo = 7 doubles per element */
 #define FNE_FLAG_ROW_GATHER 2 /* P1: one thread per CSR row gathers its incident elements (no LDS
                                  staging); inc holds uint64 items e[0:32) a[32:34) pos_b[40+8b:48+8b) */
+#define FNE_FLAG_GL_PRODUCTS 32 /* Ginzburg-Landau: qdata carries the quadrature products after w; with
+                                   nq == 3 the kernels evaluate energy, gradient and Hessian in one
+                                   fused-multiply-add pass over them */
 
 typedef struct fne_plan fne_plan;
 

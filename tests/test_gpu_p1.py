@@ -137,6 +137,25 @@ def test_fused_energy_grad_hessian_matches_separate_calls(require_gpu, amode):
     assert abs(e_fused - ref["e"]) <= 1e-13 * abs(ref["e"])
 
 
+@pytest.mark.parametrize("amode", ["patch", "staged", "rowgather"])
+def test_fused_energy_counts_the_load_term(require_gpu, amode):
+    """p-Laplace with f = -10: the fused pass's energy includes -f.u (each free row once), equal to
+    the separate energy call and the oracle; repeated fused calls give the same bits (the in-kernel
+    finalisation merges the workgroup pairs in index order and re-arms its counter)."""
+    d, adj, _ = build_plaplace_problem(40)
+    asm = PLaplaceReorderedElementAssembler(d, None, adj, hessian_precision="fp64", assembly_mode=amode)
+    u = np.random.default_rng(11).uniform(0.0, 1.0, d["freedofs"].size)
+    x = asm.create_vec(u[asm.layout.perm])
+    g = x.duplicate()
+    e_sep = float(asm.energy_device(x.t).item())
+    e1 = float(asm.energy_grad_hessian_device(x.t, g.t).item())
+    e2 = float(asm.energy_grad_hessian_device(x.t, g.t).item())
+    ref = assembly.assemble("plaplace", d, u)
+    assert abs(e1 - ref["e"]) <= 1e-13 * abs(ref["e"])
+    assert abs(e1 - e_sep) <= 1e-14 * abs(e_sep)
+    assert e1 == e2
+
+
 @pytest.mark.parametrize("family", ["plaplace", "gl"])
 @pytest.mark.parametrize("amode", ["patch", "staged", "rowgather"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

@@ -123,3 +123,22 @@ def test_gl_quadrature_table_is_exact_p1_rule():
     # int phi_a phi_b over the reference triangle (area-normalised): (1 + delta_ab) / 12
     M = np.einsum("aq,bq,q->ab", ip, ip, w)
     assert np.allclose(M, (np.ones((3, 3)) + np.eye(3)) / 12.0)
+
+
+def test_gl_table_carries_the_hessian_quadrature_products():
+    """The Ginzburg-Landau kernel table: ip, w, then w_q ip[a][q] ip[b][q] for ab = 00 11 22 01 02
+    12.  Summed over q these are the P1 mass matrix entries (1 + delta_ab) / 12, the closed form
+    the one-pass 3-point kernels rely on."""
+    from fenics_nonlinear_energies_amd.problems.ginzburg_landau import GLReorderedElementAssembler
+
+    d, _ = fixtures.gl(2)
+    asm = GLReorderedElementAssembler.__new__(GLReorderedElementAssembler)
+    asm.params = d
+    t = asm._quadrature_table()
+    nq = d["w"].size
+    assert t.size == 10 * nq
+    assert np.array_equal(t[: 3 * nq], np.asarray(d["ip"]).reshape(-1))
+    assert np.array_equal(t[3 * nq : 4 * nq], d["w"])
+    prod = t[4 * nq :].reshape(6, nq).sum(axis=1)
+    mass = np.array([2, 2, 2, 1, 1, 1]) / 12.0
+    assert np.allclose(prod, mass, rtol=1e-14, atol=0)
