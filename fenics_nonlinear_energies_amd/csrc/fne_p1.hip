@@ -512,6 +512,10 @@ __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, cons
     __syncthreads();
     if (has_row) {
       double gacc = 0.0;
+      // the row's own column (b == a: its diagonal entry, hit by every incidence) accumulates in
+      // a register in the same order; the two other columns go through the LDS row
+      HT diag = HT(0);
+      int pdiag = 31;
       auto consume = [&](uint32_t it) {
         const int s = int(it & 0x1FFFu);
         const int a = int((it >> 13) & 3u);
@@ -524,18 +528,21 @@ __global__ __launch_bounds__(kBlock) void p1_staged_kernel(fne_plan_desc d, cons
             pos[b] = int((it >> (15 + 5 * b)) & 31u);
             h[b] = sH[sym3(a, b) * cap + s];
           }
+          diag += a == 0 ? h[0] : (a == 1 ? h[1] : h[2]);
+          pdiag = a == 0 ? pos[0] : (a == 1 ? pos[1] : pos[2]);
 #pragma unroll
           for (int b = 0; b < 3; ++b)
-            if (pos[b] != 31) old[b] = sVal[roff + pos[b]];
+            if (b != a && pos[b] != 31) old[b] = sVal[roff + pos[b]];
 #pragma unroll
           for (int b = 0; b < 3; ++b)
-            if (pos[b] != 31) sVal[roff + pos[b]] = old[b] + h[b];
+            if (b != a && pos[b] != 31) sVal[roff + pos[b]] = old[b] + h[b];
         }
       };
 #pragma unroll
       for (int k = 0; k < kStagedPrefetch; ++k)
         if (k < cnt) consume(items[k]);
       for (int k = kStagedPrefetch; k < cnt; ++k) consume(d.inc[ib + k]);
+      if (WANT_H && pdiag != 31) sVal[roff + pdiag] = diag;
       if (WANT_G) g[r] = f ? gacc - f[r] : gacc;
     }
     if (WANT_H) {
