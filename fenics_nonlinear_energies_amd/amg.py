@@ -178,6 +178,21 @@ def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
     return (V / w) @ V.T, True
 
 
+def coarse_inverse_device(S: torch.Tensor) -> tuple[torch.Tensor, bool]:
+    """``coarse_inverse`` on the GPU, in FP64: Cholesky and its inverse
+    (``torch.linalg.cholesky_ex`` / ``torch.cholesky_inverse``) when the symmetric operator ``S``
+    is positive definite, else the pseudo-inverse over its eigenvalues above 1e-12 of the largest
+    |eigenvalue| (``torch.linalg.eigh``).  One host read (the factorisation's status) per call.
+    Returns (inverse, whether the fallback was taken)."""
+    L, info = torch.linalg.cholesky_ex(S)
+    if int(info.item()) == 0:
+        return torch.cholesky_inverse(L), False
+    w, V = torch.linalg.eigh(S)
+    cut = 1e-12 * w.abs().max()
+    winv = torch.where(w > cut, 1.0 / torch.where(w > cut, w, torch.ones_like(w)), torch.zeros_like(w))
+    return (V * winv) @ V.T, True
+
+
 def _injection_rows(P: sp.csr_matrix) -> np.ndarray:
     """For each column j of an interpolation P, a row whose only entry is P[i, j] = 1 (the fine
     DOF that coincides with coarse DOF j); raises if a column has none."""
@@ -360,6 +375,15 @@ class SmoothedAggregationPC:
         cols = np.tile(np.arange(nc), nc)
         self.coarse_op = _DevMat(np.arange(0, nc * nc + 1, nc), cols, nc, device, "fp64")
         self._coarse_n = nc
+        # the coarsest operator's CSR entries -> positions in its dense row-major copy
+        rows = np.repeat(np.arange(nc, dtype=np.int64), np.diff(A_pat.indptr))
+        self._coarse_flat = torch.as_tensor(rows * nc + A_pat.indices.astype(np.int64)).to(device)
+        if torch.device(device).type == "cuda":
+            # first use of the device dense solvers (library handles, kernel loading), here in the
+            # symbolic phase rather than in the first numeric set-up
+            e = torch.eye(4, dtype=torch.float64, device=device)
+            coarse_inverse_device(e)
+            torch.linalg.eigh(e)
         self.levels = levels
         self.symbolic_time += time.perf_counter() - t0
 
@@ -476,13 +500,20 @@ class SmoothedAggregationPC:
             if lvl + 1 < len(self.levels):
                 self.levels[lvl + 1]["A"].data.copy_(Ac64.data)
             a_ptr, a_idx, a_dat = Ac64.indptr, Ac64.indices, Ac64.data
-        # coarsest level: dense inverse on the host (Cholesky; positive-part pseudo-inverse if
-        # the operator is not positive definite)
+        # coarsest level: dense FP64 inverse (Cholesky; positive-part pseudo-inverse if the
+        # operator is not positive definite), on the GPU when the hierarchy is there
         nc, cp = self._coarse_n, self._coarse_pat
-        Ad = sp.csr_matrix((a_dat.double().cpu().numpy(), cp.indices, cp.indptr), shape=(nc, nc)).toarray()
-        Ainv, fallback = coarse_inverse(0.5 * (Ad + Ad.T))
+        if self.coarse_op.data.device.type == "cuda":
+            Ad = torch.zeros(nc * nc, dtype=torch.float64, device=self.coarse_op.data.device)
+            Ad.index_copy_(0, self._coarse_flat, a_dat.double())
+            Ad = Ad.view(nc, nc)
+            Ainv, fallback = coarse_inverse_device(0.5 * (Ad + Ad.T))
+            self.coarse_op.data.copy_(Ainv.reshape(-1))
+        else:
+            Ad = sp.csr_matrix((a_dat.double().cpu().numpy(), cp.indices, cp.indptr), shape=(nc, nc)).toarray()
+            Ainv, fallback = coarse_inverse(0.5 * (Ad + Ad.T))
+            self.coarse_op.data.copy_(torch.as_tensor(np.ascontiguousarray(Ainv).ravel()))
         self.coarse_eig_fallbacks += int(fallback)
-        self.coarse_op.data.copy_(torch.as_tensor(np.ascontiguousarray(Ainv).ravel()))
         self.setup_count += 1
         self.setup_time += time.perf_counter() - t_setup
         if self.use_graph and self._graph is None and torch.device(device).type == "cuda":

@@ -182,3 +182,28 @@ def test_chebyshev_coefficients_give_the_chebyshev_error_polynomial(steps):
     np.testing.assert_allclose(err, expect, atol=1e-12)
     with pytest.raises(ValueError):
         amg.SmoothedAggregationPC(smoother="sor")
+
+
+@pytest.mark.parametrize("case", ["spd", "indefinite", "negative_trace"])
+def test_coarse_inverse_device_form_equals_the_host_form(case):
+    """The set-up's tensor form of the coarse inverse (Cholesky and its inverse, or the
+    positive-part pseudo-inverse) equals the LAPACK form on the same operator and takes the
+    fallback on the same operators; run here on CPU tensors, on the GPU in the set-up."""
+    import torch
+
+    from fenics_nonlinear_energies_amd.amg import coarse_inverse, coarse_inverse_device
+
+    rng = np.random.default_rng(2)
+    n = 50
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.geomspace(1e-3, 10.0, n)
+    if case == "indefinite":
+        lam[:3] = [-2.0, -0.5, 0.0]
+    elif case == "negative_trace":
+        lam = np.concatenate((-np.geomspace(1.0, 50.0, n - 5), np.geomspace(1e-2, 1.0, 5)))
+    A = (q * lam) @ q.T
+    A = 0.5 * (A + A.T)
+    ref, fb_ref = coarse_inverse(A)
+    got, fb = coarse_inverse_device(torch.as_tensor(A))
+    assert fb == fb_ref == (case != "spd")
+    assert np.allclose(got.numpy(), ref, rtol=1e-8, atol=1e-9 * np.abs(ref).max())
