@@ -162,8 +162,7 @@ def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
     Cholesky (LAPACK potrf/potri) when the operator is positive definite; otherwise the
     pseudo-inverse over its eigenvalues above 1e-12 max|eigenvalue| (a positive cutoff whatever
     the sign of the trace), which keeps the coarse correction positive semi-definite when the
-    Hessian is indefinite (the eigendecomposition runs through torch's LAPACK, about twice as
-    fast as numpy's here).  Returns (inverse, whether the fallback was taken).
+    Hessian is indefinite.  Returns (inverse, whether the fallback was taken).
     """
     nc = Ad.shape[0]
     c, info = sla.lapack.dpotrf(Ad, lower=1)
@@ -171,15 +170,16 @@ def coarse_inverse(Ad: np.ndarray) -> tuple[np.ndarray, bool]:
         Ainv, info = sla.lapack.dpotri(c, lower=1)  # lower triangle of the inverse
     if info == 0:
         return np.tril(Ainv) + np.tril(Ainv, -1).T, False
-    w, V = torch.linalg.eigh(torch.from_numpy(np.ascontiguousarray(Ad, dtype=np.float64)))
-    w, V = w.numpy(), V.numpy()
+    # the same LAPACK as the Cholesky above: mixing it with torch's CPU thread pool right after a
+    # failed factorisation oversubscribed the host and cost ~0.1 s per fallback
+    w, V = sla.eigh(np.ascontiguousarray(Ad, dtype=np.float64))
     cut = 1e-12 * float(np.abs(w).max()) if w.size else 0.0
     w = np.where(w > cut, w, np.inf)
     return (V / w) @ V.T, True
 
 
 def coarse_inverse_device(S: torch.Tensor) -> tuple[torch.Tensor, bool]:
-    """``coarse_inverse`` on the GPU, in FP64: Cholesky and its inverse
+    """``coarse_inverse`` in FP64 on the tensor's device: Cholesky and its inverse
     (``torch.linalg.cholesky_ex`` / ``torch.cholesky_inverse``) when the symmetric operator ``S``
     is positive definite, else the pseudo-inverse over its eigenvalues above 1e-12 of the largest
     |eigenvalue| (``torch.linalg.eigh``).  One host read (the factorisation's status) per call.
@@ -189,7 +189,8 @@ def coarse_inverse_device(S: torch.Tensor) -> tuple[torch.Tensor, bool]:
         return torch.cholesky_inverse(L), False
     w, V = torch.linalg.eigh(S)
     cut = 1e-12 * w.abs().max()
-    winv = torch.where(w > cut, 1.0 / torch.where(w > cut, w, torch.ones_like(w)), torch.zeros_like(w))
+    keep = w > cut
+    winv = torch.where(keep, 1.0 / torch.where(keep, w, torch.ones_like(w)), torch.zeros_like(w))
     return (V * winv) @ V.T, True
 
 
@@ -260,14 +261,21 @@ class SmoothedAggregationPC:
     def __init__(self, near_nullspace: np.ndarray | None = None, *, agg_dofs: int | tuple | None = None,
                  max_coarse: int = 1200, max_levels: int = 8, sweeps: int = 2, precision: str = "fp64",
                  use_graph: bool = True, fixed_prolongators: list | None = None,
-                 coarse_near_nullspace: np.ndarray | None = None, smoother: str = "chebyshev"):
+                 coarse_near_nullspace: np.ndarray | None = None, smoother: str = "chebyshev",
+                 coarse_factor: str = "host"):
         """``fixed_prolongators``: host CSR transfers (n_l x n_{l+1}) of the finest levels, used as
         given (no aggregation, no smoothing; e.g. the same-mesh P4 -> P2 -> P1 interpolations of
         ``pmg.py``); aggregation continues below the last of them with ``coarse_near_nullspace``
         (rows of the coarsest fixed level; default: the near-nullspace injected through the
-        transfers' unit rows)."""
+        transfers' unit rows).  ``coarse_factor``: where the coarsest operator's dense FP64 inverse
+        is computed at set-up, "host" (LAPACK) or "device" (the GPU's dense solvers, no host
+        round trip); the cycle applies it on the device either way.  The two agree to rounding;
+        the inexact Newton paths built on them can differ by an iteration or two."""
         if smoother not in ("jacobi", "chebyshev"):
             raise ValueError(f"unsupported multigrid smoother {smoother!r} (jacobi | chebyshev)")
+        if coarse_factor not in ("host", "device"):
+            raise ValueError(f"coarse_factor must be 'host' or 'device', not {coarse_factor!r}")
+        self.coarse_factor = coarse_factor
         # level smoother: damped Jacobi (omega = 4 / (3 rho)), or Chebyshev on [0.1, 1.1] rho with
         # Jacobi scaling (PETSc's KSPCHEBYSHEV eigenvalue bounds, the default PCGAMG smoother)
         self.smoother = smoother
@@ -375,15 +383,21 @@ class SmoothedAggregationPC:
         cols = np.tile(np.arange(nc), nc)
         self.coarse_op = _DevMat(np.arange(0, nc * nc + 1, nc), cols, nc, device, "fp64")
         self._coarse_n = nc
-        # the coarsest operator's CSR entries -> positions in its dense row-major copy
-        rows = np.repeat(np.arange(nc, dtype=np.int64), np.diff(A_pat.indptr))
-        self._coarse_flat = torch.as_tensor(rows * nc + A_pat.indices.astype(np.int64)).to(device)
-        if torch.device(device).type == "cuda":
-            # first use of the device dense solvers (library handles, kernel loading), here in the
-            # symbolic phase rather than in the first numeric set-up
-            e = torch.eye(4, dtype=torch.float64, device=device)
-            coarse_inverse_device(e)
-            torch.linalg.eigh(e)
+        # first use of the dense solvers of the coarse inverse at the coarse size (library
+        # loading, thread pools, device solver handles), here in the symbolic phase rather than in
+        # whichever numeric set-up first needs them: a definite operator (Cholesky) and an
+        # indefinite one (the eigendecomposition fallback)
+        warm = np.diag(np.linspace(1.0, 2.0, nc))
+        warm_indef = warm.copy()
+        warm_indef[0, 0] = -1.0
+        if self.coarse_factor == "device" and torch.device(device).type == "cuda":
+            rows = np.repeat(np.arange(nc, dtype=np.int64), np.diff(A_pat.indptr))
+            self._coarse_flat = torch.as_tensor(rows * nc + A_pat.indices.astype(np.int64)).to(device)
+            coarse_inverse_device(torch.as_tensor(warm, device=device))
+            coarse_inverse_device(torch.as_tensor(warm_indef, device=device))
+        else:
+            coarse_inverse(warm)
+            coarse_inverse(warm_indef)
         self.levels = levels
         self.symbolic_time += time.perf_counter() - t0
 
@@ -501,9 +515,9 @@ class SmoothedAggregationPC:
                 self.levels[lvl + 1]["A"].data.copy_(Ac64.data)
             a_ptr, a_idx, a_dat = Ac64.indptr, Ac64.indices, Ac64.data
         # coarsest level: dense FP64 inverse (Cholesky; positive-part pseudo-inverse if the
-        # operator is not positive definite), on the GPU when the hierarchy is there
+        # operator is not positive definite), on the host or the device, applied on the device
         nc, cp = self._coarse_n, self._coarse_pat
-        if self.coarse_op.data.device.type == "cuda":
+        if getattr(self, "_coarse_flat", None) is not None:
             Ad = torch.zeros(nc * nc, dtype=torch.float64, device=self.coarse_op.data.device)
             Ad.index_copy_(0, self._coarse_flat, a_dat.double())
             Ad = Ad.view(nc, nc)

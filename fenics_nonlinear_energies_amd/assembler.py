@@ -204,13 +204,18 @@ class ReorderedElementAssemblerBase:
         """The multigrid level-smoother options among ``pc_options`` (PETSc option names, as the
         reference passes them): ``mg_levels_ksp_type`` ("chebyshev", the default as in PCGAMG,
         or "richardson" = damped Jacobi sweeps) and ``mg_levels_ksp_max_it`` (steps per level and
-        side, 2 by default).  Other options have no counterpart on this solver stack and are
-        ignored."""
+        side, 2 by default); and this build's ``mg_coarse_factor`` ("host": LAPACK, the default,
+        or "device": the GPU's dense solvers) for the coarsest level's inverse.  Other options
+        have no counterpart on this solver stack and are ignored."""
         opts = {str(k).lstrip("-"): v for k, v in dict(pc_options or {}).items()}
         kind = str(opts.get("mg_levels_ksp_type", "chebyshev")).lower()
         if kind not in ("richardson", "jacobi", "chebyshev"):
             raise ValueError(f"unsupported mg_levels_ksp_type {kind!r} (richardson | chebyshev)")
         self.ksp.mg_smoother = "chebyshev" if kind == "chebyshev" else "jacobi"
+        coarse = str(opts.get("mg_coarse_factor", "host")).lower()
+        if coarse not in ("host", "device"):
+            raise ValueError(f"unsupported mg_coarse_factor {coarse!r} (host | device)")
+        self.ksp.mg_coarse_factor = coarse
         self._pc_sweeps_given = "mg_levels_ksp_max_it" in opts
         if self._pc_sweeps_given:
             self.ksp.mg_sweeps = int(opts["mg_levels_ksp_max_it"])

@@ -31,12 +31,13 @@ def free_node_table(freedofs: np.ndarray, perm: np.ndarray, block: int):
     return uniq[full], idx
 
 
-def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it, ksp_type="cg", pc_type="gamg"):
+def _he_assembler(nx, ny, nz, E, nu, precision, ksp_max_it, ksp_type="cg", pc_type="gamg", coarse_factor="host"):
     from .problems.hyperelasticity import HEReorderedElementAssembler, build_beam_problem
 
     p, adj, _ = build_beam_problem(nx, ny, nz, E=E, nu=nu)
     asm = HEReorderedElementAssembler(p, None, adj, hessian_precision=precision, ksp_type=ksp_type,
-                                      pc_type=pc_type, ksp_rtol=1e-1, ksp_max_it=ksp_max_it)
+                                      pc_type=pc_type, ksp_rtol=1e-1, ksp_max_it=ksp_max_it,
+                                      pc_options={"mg_coarse_factor": coarse_factor})
     return p, asm
 
 
@@ -56,7 +57,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
                        ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=200, lr=2e-3,
                        seed=0, guard=True, with_reference=True, with_trust_reference=False, ksp_forcing=None,
-                       newton_kwargs=None, log=print):
+                       newton_kwargs=None, coarse_factor="device", log=print):
     """Load-step time to tolerance on ``eval_mesh``: reference profiles (FP64) against previous /
     extrapolated / learned initial guesses (FP32 Hessian), after training the nodal MLP on runs
     of ``train_mesh``.
@@ -67,6 +68,11 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     a line search on [-0.5, 2] (tol 1e-1) inside the radius, multigrid rebuilt at every Newton
     iteration, KSP rtol 1e-1 capped at 30 iterations; stopping tolerances as for the others.
     It also adds ``learned_trust_fp32``: the same settings from the learned guess in FP32.
+
+    ``coarse_factor`` (every variant alike): where the multigrid's coarsest level is factorised
+    at each set-up.  "device" keeps the set-up on the GPU; with "host" a few of the host LAPACK
+    calls of a run took 35-120 ms instead of ~2 ms on the GPU box (thread start-up competing
+    with the other OpenBLAS pool of the process), a random cost of ~0.1-0.2 s per run.
     """
     from .problems.hyperelasticity import solve_he_load_steps
 
@@ -75,7 +81,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
     feats, targs = [], []
     t_gen = time.perf_counter()
     for (E, nu) in train_materials:
-        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it, ksp_type, pc_type)
+        p, asm = _he_assembler(*train_mesh, E, nu, "fp32", ksp_max_it, ksp_type, pc_type, coarse_factor)
         pred = he_predictor(p, asm, None, "extrapolate", total_angle)
         holder = {}
         recs, _ = solve_he_load_steps(asm, total_steps, total_angle, predictor=pred, ksp_max_it=ksp_max_it,
@@ -115,7 +121,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         # previous state, FP64, KSP capped at 30 iterations, PC kept and rebuilt at the cap
         variants.insert(0, ("reference_fp64", "previous", "fp64", True, None, 30, ksp_type, {}))
     for name, mode, prec, reuse, refresh, max_it, v_ksp, v_newton in variants:
-        p, asm = _he_assembler(*eval_mesh, E, nu, prec, max_it, v_ksp, pc_type)
+        p, asm = _he_assembler(*eval_mesh, E, nu, prec, max_it, v_ksp, pc_type, coarse_factor)
         pred = he_predictor(p, asm, net, mode, total_angle)
         pred.feat_mean, pred.feat_std, pred.targ_std = stats
         if guard:

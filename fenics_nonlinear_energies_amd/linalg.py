@@ -273,6 +273,7 @@ class KSP:
         self.mg_coarse_near_nullspace = None
         self.mg_sweeps = 2  # smoothing steps per level, each side (-mg_levels_ksp_max_it)
         self.mg_smoother = "chebyshev"  # "chebyshev" | "jacobi" (-mg_levels_ksp_type chebyshev | richardson)
+        self.mg_coarse_factor = "host"  # where the coarsest level is factorised: "host" | "device"
 
     # PETSc-style configuration -----------------------------------------------------------
     def setType(self, t: str) -> None:
@@ -353,9 +354,10 @@ class KSP:
             return SmoothedAggregationPC(self.near_nullspace, precision=precision,
                                          fixed_prolongators=self.mg_prolongators,
                                          coarse_near_nullspace=self.mg_coarse_near_nullspace,
-                                         sweeps=self.mg_sweeps, smoother=self.mg_smoother)
+                                         sweeps=self.mg_sweeps, smoother=self.mg_smoother,
+                                         coarse_factor=self.mg_coarse_factor)
         return SmoothedAggregationPC(self.near_nullspace, precision=precision, sweeps=self.mg_sweeps,
-                                     smoother=self.mg_smoother)
+                                     smoother=self.mg_smoother, coarse_factor=self.mg_coarse_factor)
 
     def prepare_pc(self, A: DeviceCSR) -> float:
         """Mesh preprocessing for the multigrid preconditioners: the symbolic phase (patterns of

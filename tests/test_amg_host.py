@@ -186,9 +186,9 @@ def test_chebyshev_coefficients_give_the_chebyshev_error_polynomial(steps):
 
 @pytest.mark.parametrize("case", ["spd", "indefinite", "negative_trace"])
 def test_coarse_inverse_device_form_equals_the_host_form(case):
-    """The set-up's tensor form of the coarse inverse (Cholesky and its inverse, or the
-    positive-part pseudo-inverse) equals the LAPACK form on the same operator and takes the
-    fallback on the same operators; run here on CPU tensors, on the GPU in the set-up."""
+    """The tensor form of the coarse inverse (``coarse_factor="device"``: Cholesky and its
+    inverse, or the positive-part pseudo-inverse) equals the LAPACK form on the same operator and
+    takes the fallback on the same operators; run here on CPU tensors, on the GPU in set-ups."""
     import torch
 
     from fenics_nonlinear_energies_amd.amg import coarse_inverse, coarse_inverse_device
