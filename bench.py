@@ -15,7 +15,8 @@ vertex coordinates), ``secondary`` (assembly and SpMV of the P4 Mohr-Coulomb slo
 Neo-Hookean tet beams at 61,440 and 491,520 elements), ``time_to_tol`` (the metric's second
 half: Newton time to tolerance on the Neo-Hookean load path, the reference's solver profile in
 FP64 against FP32 Hessians from the learned warm start, network trained in the same run),
-``roofline`` and ``cpu_baseline``.
+``time_to_tol_headline`` (the same on the headline workload: trust-region Newton with FP64 against FP32
+Hessians, no warm start), ``roofline`` and ``cpu_baseline``.
 
 Prints one JSON line (see the field comments below).  ``--gpus N`` with N > 1 exits: this
 build targets one GPU.
@@ -52,7 +53,8 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true", help="skip the P4 slope and tet beam assembly lines")
     ap.add_argument("--no-fp64-line", action="store_true", help="skip the FP64-Hessian line of the headline workload")
     ap.add_argument("--no-time-to-tol", action="store_true",
-                    help="skip the Neo-Hookean load-path time to tolerance (plain FP64 vs mixed + learned)")
+                    help="skip the times to tolerance (Neo-Hookean load path: plain FP64 vs mixed + learned; "
+                         "the headline workload: FP64 vs FP32 Hessians)")
     ap.add_argument(
         "--copy-calib-mib", type=int, default=0,
         help="before warm-up, run 3 device copies of this many MiB (counter calibration reference)",
@@ -211,6 +213,29 @@ def time_to_tol_beam() -> dict:
         "training": {"samples": out["train_samples"], "loss_first": out["loss_first"], "loss_last": out["loss_last"],
                      "train_s": out["train_time_s"], "data_runs_s": out["data_gen_time_s"]},
         "study_total_s": total,
+    }
+
+
+def time_to_tol_gl(n_cells: int) -> dict:
+    """Newton time to tolerance on the headline workload itself (Ginzburg-Landau n x n, its
+    sine-plus-noise start, seed 0): trust-region Newton with Steihaug CG and the multigrid, FP64
+    Hessians (the plain FP64 form) against FP32 Hessian values and Krylov products (energy,
+    gradient and residual FP64), both after a warm-up on a small grid."""
+    from fenics_nonlinear_energies_amd.experiments import gl_precision_study
+
+    t0 = time.perf_counter()
+    out = gl_precision_study(n=n_cells, seed=0, log=lambda *a: None)
+    v64, v32 = out["variants"]["fp64"], out["variants"]["fp32"]
+    return {
+        "workload": f"Ginzburg-Landau {n_cells}x{n_cells} P1 ({out['n_free']} free DOFs), trust-region Newton + "
+                    "Steihaug CG + multigrid, to |g| <= 1e-6",
+        "plain_fp64_s": v64["wall_s"],
+        "mixed_fp32_s": v32["wall_s"],
+        "speedup": out["speedup"],
+        "final_energy_rel_diff": out["energy_rel_diff"],
+        "variants": {k: {x: v[x] for x in ("wall_s", "newton_its", "linear_its", "energy", "message")}
+                     for k, v in out["variants"].items()},
+        "study_total_s": time.perf_counter() - t0,
     }
 
 
@@ -392,6 +417,7 @@ def main():
         out["secondary"] = secondary_workloads()
     if not args.no_time_to_tol:
         out["time_to_tol"] = time_to_tol_beam()
+        out["time_to_tol_headline"] = time_to_tol_gl(args.n)
     if not args.no_cpu_baseline:
         single = cpu_baseline(args.cpu_sample_n, args.cpu_budget_s)
         omp = cpu_baseline_omp(args.n, args.cpu_budget_s)

@@ -274,3 +274,54 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
         if base is not None:
             v["speedup_vs_reference_fp64"] = base["wall_s"] / v["wall_s"]
     return out
+
+
+def gl_precision_study(*, n=1024, seed=0, ksp_type="stcg", pc_type="gamg", ksp_rtol=1e-3, ksp_max_it=1000,
+                       trust_radius_init=1.0, newton_kwargs=None, coarse_factor="device", log=print):
+    """The Ginzburg-Landau configuration end to end on ``n`` x ``n`` cells: trust-region Newton
+    (Steihaug CG subproblems with the multigrid, golden-section search inside the radius) from
+    0.5 sin(pi x) sin(pi y) + U[-1e-3, 1e-3] (``seed``), once with FP64 Hessian values and once
+    with FP32 Hessian values and Krylov products (energy, gradient and Newton residual FP64 in
+    both).  Returns per variant the wall time to tolerance, Newton and Krylov iterations and the
+    final energy, and the FP32 run's relative energy difference."""
+    from .drivers import solve_newton
+    from .problems.ginzburg_landau import GLReorderedElementAssembler, build_gl_problem, sine_noise_init
+
+    nk = dict(tolf=1e-9, tolg=1e-6, tolg_rel=0.0, tolx_rel=1e-6, tolx_abs=1e-10, maxit=100)
+    nk.update(newton_kwargs or {})
+    # warm-up on a 32^2 grid in both precisions: the first launches of the kernels, the graph
+    # capture path and the dense solvers' start-up are not charged to either timed run
+    wp, wadj = build_gl_problem(32)
+    for prec in ("fp64", "fp32"):
+        w = GLReorderedElementAssembler(wp, None, wadj, ksp_type=ksp_type, pc_type=pc_type, ksp_rtol=ksp_rtol,
+                                        ksp_max_it=ksp_max_it, hessian_precision=prec,
+                                        pc_options={"mg_coarse_factor": coarse_factor})
+        solve_newton(w, w.create_vec(sine_noise_init(wp, seed)[w.layout.perm]), ksp_rtol=ksp_rtol,
+                     ksp_max_it=ksp_max_it, use_trust_region=True, trust_radius_init=trust_radius_init,
+                     trust_subproblem_line_search=True, linesearch_tol=1e-3, **nk)
+        w.cleanup()
+    params, adj = build_gl_problem(n)
+    u0 = sine_noise_init(params, seed)
+    out = {"n_elem": int(params["elems"].shape[0]), "n_free": int(params["freedofs"].size), "variants": {}}
+    for prec in ("fp64", "fp32"):
+        asm = GLReorderedElementAssembler(params, None, adj, ksp_type=ksp_type, pc_type=pc_type, ksp_rtol=ksp_rtol,
+                                          ksp_max_it=ksp_max_it, hessian_precision=prec,
+                                          pc_options={"mg_coarse_factor": coarse_factor})
+        x = asm.create_vec(u0[asm.layout.perm])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, use_trust_region=True,
+                           trust_radius_init=trust_radius_init, trust_subproblem_line_search=True,
+                           linesearch_tol=1e-3, **nk)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        amg = getattr(asm.ksp, "_amg", None)
+        out["variants"][prec] = {"wall_s": wall, "newton_its": int(res["nit"]), "linear_its": int(res["linear_iters"]),
+                                 "message": res["message"], "energy": float(res["fun"]),
+                                 "pc_setups": int(getattr(amg, "setup_count", 0)) if amg is not None else 0}
+        log(f"{prec}: {res['message']} nit={res['nit']} lin={res['linear_iters']} E={res['fun']!r} wall={wall:.2f}s")
+        asm.cleanup()
+    v64, v32 = out["variants"]["fp64"], out["variants"]["fp32"]
+    out["energy_rel_diff"] = abs(v32["energy"] - v64["energy"]) / abs(v64["energy"])
+    out["speedup"] = v64["wall_s"] / v32["wall_s"]
+    return out
