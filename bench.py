@@ -240,8 +240,8 @@ def time_to_tol_gl(n_cells: int) -> dict:
 
 
 def secondary_workloads() -> list[dict]:
-    """Fused Newton-step assembly and SpMV on the P4 slope (C5 size) and the tet beams (C3 and C4
-    sizes)."""
+    """Fused Newton-step assembly and SpMV on the P4 slope (h = 1/7, 1.57M DOFs) and the tet beams
+    (160x8x8 and 320x16x16 hexes x 6 tets)."""
     import numpy as np
 
     from fenics_nonlinear_energies_amd.problems import slope_stability as S

@@ -194,7 +194,7 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
     ``pc_reuse``: whether the non-reference variants keep the preconditioner between Newton
     iterations (rebuilding it at the KSP cap); None: keep it for "gamg", set it up at every
     iteration for "mg", whose device set-up costs less than the CG iterations a stale
-    p-multigrid adds (with reuse the C5 continuation took 15,366 CG iterations against 11,114).
+    p-multigrid adds (with reuse the 1.57M-DOF h = 1/7 continuation took 15,366 CG iterations against 11,114).
 
     ``mixed="fp32pc"``: FP64 Hessian and Krylov operator, multigrid stored in FP32 (the P4
     nu = 0.49 Hessian is too ill-conditioned for FP32 storage at ~1.5M DOFs: rounding its

@@ -123,7 +123,7 @@ def test_p4_slope_c5_size_matrix_free_product_symmetry_gradient(require_gpu):
     asm.A.mult_tensor(d, y_csr)
     asm.hvp_device(x.t, d, y_mf)
     scale = float(y_csr.abs().max())
-    print(f"P4 C5: |H d - hvp d|_max / |H d|_max = {float((y_csr - y_mf).abs().max()) / scale:.3e}", flush=True)
+    print(f"P4 slope h=1/7: |H d - hvp d|_max / |H d|_max = {float((y_csr - y_mf).abs().max()) / scale:.3e}", flush=True)
     assert float((y_csr - y_mf).abs().max()) <= 1e-10 * scale
     H = asm.A.to_scipy()
     assert abs(H - H.T).max() <= 1e-12 * abs(H).max()
