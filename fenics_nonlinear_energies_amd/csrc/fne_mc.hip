@@ -380,14 +380,19 @@ template <> struct Pair<double> {
   static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
 };
 
-// Row gather.  A group of 16 lanes owns kMcRowBatch consecutive CSR rows; lane c < NL/2 handles
+// Row gather.  A group of 16 lanes owns mc_row_batch(NL) consecutive CSR rows; lane c < NL/2 handles
 // the element columns 2c and 2c+1 (paired loads).  inc[it] = element * 32 + local row lists the incidences of
 // every row (rows consecutive, elements ascending); pos[it*32 + c] is the position of the
 // element's local column c inside the row (255 = constrained).  The incidences of the whole
 // batch are read in one load (lane j holds item j), up to 8 items' element rows and positions
 // are fetched together, and then added in item order into LDS copies of the rows, which are
 // written once.  Every CSR entry therefore sums its contributions in a fixed order.
-constexpr int kMcRowBatch = 4;
+// rows per 16-lane group: one for 12-column element rows (P2, tets), two for the 30 of P4
+// (measured on MI355X: P4 0.50 ms with 2 rows against 0.52 with 4 and 0.55 with 1; the
+// 320x16x16 tet beam 0.335 ms with 1 row against 0.37 with 2 and 0.40 with 4)
+__host__ __device__ constexpr int mc_row_batch(int nl) {
+  return nl <= 12 ? 1 : 2;
+}
 template <int NL, typename HT, bool WG, bool WH>
 __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, const HT* __restrict__ hbuf,
                                                                const double* __restrict__ gbuf,
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
                                                                double* __restrict__ g, HT* __restrict__ vals) {
   extern __shared__ unsigned char mc_smem[];
   constexpr int kL = kMcRowLanes;
-  constexpr int kRB = kMcRowBatch;
+  constexpr int kRB = mc_row_batch(NL);
   constexpr int kRowsBlk = (kBlock / kL) * kRB;
   constexpr int kPf = 8;
   const int cap = (d.max_row_len + 1) & ~1;
@@ -560,7 +565,7 @@ int launch_mc(fne_plan* plan, FneOp op, const FneCall& c) {
     if (rc) return rc;
   }
   if (!want_g && !want_h) return FNE_OK;
-  const int rows_per_block = (kBlock / kMcRowLanes) * kMcRowBatch;
+  const int rows_per_block = (kBlock / kMcRowLanes) * mc_row_batch(NL);
   const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
   const int cap = (d.max_row_len + 1) & ~1;
   if (want_h && size_t(rows_per_block) * cap * (fp32 ? 4 : 8) > plan->max_lds) {
@@ -790,7 +795,7 @@ extern "C" int fne_hessian_diag(fne_plan* plan, const double* x, const double* d
 int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
   const bool fp32 = c.precision == FNE_FP32;
-  const int rows_per_block = (kBlock / kMcRowLanes) * kMcRowBatch;
+  const int rows_per_block = (kBlock / kMcRowLanes) * mc_row_batch(d.n_loc);
   const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
   const int cap = (d.max_row_len + 1) & ~1;
   const size_t shm = want_h ? size_t(rows_per_block) * cap * (fp32 ? 4 : 8) : 0;
