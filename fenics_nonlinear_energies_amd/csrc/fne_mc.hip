@@ -29,7 +29,11 @@ namespace {
 constexpr int kMcMaxQ = 21;  // 3 elements x 21 points fill one wavefront
 constexpr int kMcEpw = 3;    // elements per wavefront iteration
 constexpr int kMcWaves = kBlock / 64;
-constexpr int kMcRowLanes = 16;  // lanes per CSR row in the row gather
+// lanes per row group in the row gather: lane c < NL/2 carries element columns 2c and 2c+1, so
+// 8 lanes cover a 12-column element row and 16 the 30 of P4
+__host__ __device__ constexpr int mc_row_lanes(int nl) {
+  return nl <= 16 ? 8 : 16;
+}
 constexpr unsigned char kPosNone = 255;
 
 typedef float mc_f4 __attribute__((ext_vector_type(4)));
@@ -380,14 +384,14 @@ template <> struct Pair<double> {
   static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
 };
 
-// Row gather.  A group of 16 lanes owns mc_row_batch(NL) consecutive CSR rows; lane c < NL/2 handles
+// Row gather.  A group of mc_row_lanes(NL) lanes owns mc_row_batch(NL) consecutive CSR rows; lane c < NL/2 handles
 // the element columns 2c and 2c+1 (paired loads).  inc[it] = element * 32 + local row lists the incidences of
 // every row (rows consecutive, elements ascending); pos[it*32 + c] is the position of the
 // element's local column c inside the row (255 = constrained).  The incidences of the whole
 // batch are read in one load (lane j holds item j), up to 8 items' element rows and positions
 // are fetched together, and then added in item order into LDS copies of the rows, which are
 // written once.  Every CSR entry therefore sums its contributions in a fixed order.
-// rows per 16-lane group: one for 12-column element rows (P2, tets), two for the 30 of P4
+// rows per lane group: one for 12-column element rows (P2, tets), two for the 30 of P4
 // (measured on MI355X: P4 0.50 ms with 2 rows against 0.52 with 4 and 0.55 with 1; the
 // 320x16x16 tet beam 0.335 ms with 1 row against 0.37 with 2 and 0.40 with 4)
 __host__ __device__ constexpr int mc_row_batch(int nl) {
@@ -399,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
                                                                const double* __restrict__ f,
                                                                double* __restrict__ g, HT* __restrict__ vals) {
   extern __shared__ unsigned char mc_smem[];
-  constexpr int kL = kMcRowLanes;
+  constexpr int kL = mc_row_lanes(NL);
   constexpr int kRB = mc_row_batch(NL);
   constexpr int kRowsBlk = (kBlock / kL) * kRB;
   constexpr int kPf = 8;
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void mc_row_gather_kernel(fne_plan_desc d, 
   const int c = threadIdx.x & (kL - 1);
   HT* sRows = reinterpret_cast<HT*>(mc_smem) + size_t(grp) * kRB * cap;
   const int r0 = xcd_remap(blockIdx.x, gridDim.x) * kRowsBlk + grp * kRB;
-  if (r0 >= d.n_free) return;  // uniform over the 16-lane group; no block-level barrier below
+  if (r0 >= d.n_free) return;  // uniform over the lane group; no block-level barrier below
   const int nr = min(kRB, d.n_free - r0);
   const unsigned char* pos = reinterpret_cast<const unsigned char*>(d.pitem);
   const unsigned int* inc = reinterpret_cast<const unsigned int*>(d.inc);
@@ -565,7 +569,7 @@ int launch_mc(fne_plan* plan, FneOp op, const FneCall& c) {
     if (rc) return rc;
   }
   if (!want_g && !want_h) return FNE_OK;
-  const int rows_per_block = (kBlock / kMcRowLanes) * mc_row_batch(NL);
+  const int rows_per_block = (kBlock / mc_row_lanes(NL)) * mc_row_batch(NL);
   const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
   const int cap = (d.max_row_len + 1) & ~1;
   if (want_h && size_t(rows_per_block) * cap * (fp32 ? 4 : 8) > plan->max_lds) {
@@ -795,7 +799,7 @@ extern "C" int fne_hessian_diag(fne_plan* plan, const double* x, const double* d
 int fne_launch_row_gather(fne_plan* plan, bool want_g, bool want_h, const FneCall& c) {
   const fne_plan_desc& d = plan->d;
   const bool fp32 = c.precision == FNE_FP32;
-  const int rows_per_block = (kBlock / kMcRowLanes) * mc_row_batch(d.n_loc);
+  const int rows_per_block = (kBlock / mc_row_lanes(d.n_loc)) * mc_row_batch(d.n_loc);
   const dim3 grid((d.n_free + rows_per_block - 1) / rows_per_block);
   const int cap = (d.max_row_len + 1) & ~1;
   const size_t shm = want_h ? size_t(rows_per_block) * cap * (fp32 ? 4 : 8) : 0;
