@@ -55,7 +55,8 @@ def he_predictor(p, asm, net, mode, total_angle, first_step=True):
 
 def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4), eval_mesh=(160, 8, 8),
                        total_steps=24, total_angle=4.0 * np.pi, ksp_max_it=2000, ksp_type="cg", pc_type="gamg",
-                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, hidden=128, layers=4, epochs=200, lr=2e-3,
+                       ksp_rtol=1e-1, pc_reuse=True, pc_refresh_its=20, pc_step_setup=False, hidden=128, layers=4,
+                       epochs=200, lr=2e-3,
                        seed=0, guard=True, with_reference=True, with_trust_reference=False, ksp_forcing=None,
                        newton_kwargs=None, coarse_factor="device", log=print):
     """Load-step time to tolerance on ``eval_mesh``: reference profiles (FP64) against previous /
@@ -131,6 +132,7 @@ def he_warmstart_study(*, train_materials, eval_material, train_mesh=(80, 4, 4),
         recs, x = solve_he_load_steps(
             asm, total_steps, total_angle, predictor=pred, ksp_max_it=max_it, ksp_rtol=ksp_rtol,
             pc_setup_on_ksp_cap=reuse, pc_refresh_its=refresh,
+            pc_keep_across_steps=name.startswith("reference") or not pc_step_setup,
             newton_kwargs=dict(newton_kwargs or {}, **v_newton,
                                **({} if name.startswith("reference") else {"ksp_forcing": ksp_forcing})),
             on_step=lambda r, h, nm=name: log(f"  {nm} step {r['step']}: nit={r['nit']} {r['time']:.2f}s"))

@@ -229,7 +229,8 @@ HE_NEWTON = dict(tolf=1e-4, tolg=1e-3, tolg_rel=1e-3, maxit=100, linesearch_inte
 def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * np.pi, *, num_steps=None,
                         predictor=None, on_step=None, ksp_rtol: float = 1e-1, ksp_max_it: int = 30,
                         pc_setup_on_ksp_cap: bool = False, pc_refresh_its: int | None = None,
-                        retry_on_failure=True, stop_on_fail: bool = False, newton_kwargs: dict | None = None):
+                        pc_keep_across_steps: bool = True, retry_on_failure=True, stop_on_fail: bool = False,
+                        newton_kwargs: dict | None = None):
     """Rotate the right face in ``total_steps`` equal steps and solve each step with Newton.
 
     The reference's load-step loop (``hyperelasticity/jax_petsc/solver.py:225-493``):
@@ -242,8 +243,10 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
     the converged steps, starting with the reference configuration at angle 0.  The reference
     rotates by 4 * 2 pi in total (``solver.py:225``); the angle is a parameter here.
     Preconditioner policy: set up at every Newton iteration (the reference's), or with
-    ``pc_setup_on_ksp_cap`` kept across iterations and steps and rebuilt when a solve hits
-    ``ksp_max_it`` or needs more than ``pc_refresh_its`` iterations.  ``stop_on_fail`` ends the
+    ``pc_setup_on_ksp_cap`` kept across iterations (and across steps unless
+    ``pc_keep_across_steps`` is False, which sets it up at each step's first Newton iteration)
+    and rebuilt when a solve hits ``ksp_max_it`` or needs more than ``pc_refresh_its``
+    iterations.  ``stop_on_fail`` ends the
     path at the first step that does not converge.  Returns (records, state vector).
     """
     import torch
@@ -263,7 +266,8 @@ def solve_he_load_steps(asm, total_steps: int = 24, total_angle: float = 4.0 * n
 
     def solve(vec, _angle):
         return solve_newton(asm, vec, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it,
-                            pc_setup_on_ksp_cap=pc_setup_on_ksp_cap, keep_pc=pc_setup_on_ksp_cap,
+                            pc_setup_on_ksp_cap=pc_setup_on_ksp_cap,
+                            keep_pc=pc_setup_on_ksp_cap and pc_keep_across_steps,
                             pc_refresh_its=pc_refresh_its, retry_on_failure=retry_on_failure, **opts)
 
     def notify(rec, history):
