@@ -129,3 +129,20 @@ def test_gl_sine_noise_trust_region_fp32_hessian_reaches_the_fp64_energy(require
     # <= 6 FP32-rounded element contributions per entry, summed in FP32
     assert abs(Hg - Ho).max() <= 8 * np.finfo(np.float32).eps * scale
     asm.cleanup()
+
+
+def test_gl_precision_study_both_precisions_converge_to_the_same_energy(require_gpu):
+    """The headline configuration end to end at 64 x 64 cells (the function behind the bench
+    line's ``time_to_tol_headline``): trust-region Newton with Steihaug CG and the multigrid
+    converges with FP64 and with FP32 Hessian values and Krylov products, and the FP32 run's
+    final energy is within the stated mixed-precision bound of the FP64 run's."""
+    from fenics_nonlinear_energies_amd.experiments import gl_precision_study
+
+    out = gl_precision_study(n=64, seed=0, log=lambda *a: None)
+    v64, v32 = out["variants"]["fp64"], out["variants"]["fp32"]
+    print(f"GL 64^2: fp64 {v64['message']!r} nit={v64['newton_its']} E={v64['energy']!r}; "
+          f"fp32 {v32['message']!r} nit={v32['newton_its']} E={v32['energy']!r}", flush=True)
+    assert out["n_free"] == 63 * 63
+    assert "converged" in v64["message"].lower() and "converged" in v32["message"].lower()
+    assert out["energy_rel_diff"] <= MIXED_REL
+    assert v64["pc_setups"] >= 1 and v32["pc_setups"] >= 1
