@@ -279,13 +279,19 @@ def slope_warmstart_study(*, train_materials, eval_material, lambdas, train_h=2.
 
 
 def gl_precision_study(*, n=1024, seed=0, ksp_type="stcg", pc_type="gamg", ksp_rtol=1e-3, ksp_max_it=1000,
-                       trust_radius_init=1.0, newton_kwargs=None, coarse_factor="device", log=print):
+                       trust_radius_init=1.0, newton_kwargs=None, coarse_factor="device", pc_refresh_its=5,
+                       log=print):
     """The Ginzburg-Landau configuration end to end on ``n`` x ``n`` cells: trust-region Newton
     (Steihaug CG subproblems with the multigrid, golden-section search inside the radius) from
-    0.5 sin(pi x) sin(pi y) + U[-1e-3, 1e-3] (``seed``), once with FP64 Hessian values and once
-    with FP32 Hessian values and Krylov products (energy, gradient and Newton residual FP64 in
-    both).  Returns per variant the wall time to tolerance, Newton and Krylov iterations and the
-    final energy, and the FP32 run's relative energy difference."""
+    0.5 sin(pi x) sin(pi y) + U[-1e-3, 1e-3] (``seed``), with FP64 Hessian values ("fp64") and
+    with FP32 Hessian values and Krylov products ("fp32"; energy, gradient and Newton residual
+    FP64 in both), the multigrid set up at every Newton iteration as in the reference's runs.
+    With ``pc_refresh_its`` two more variants keep the multigrid across Newton iterations and
+    rebuild it after a Steihaug CG solve that needed more than that many iterations
+    ("fp64_reuse", "fp32_reuse").  Returns per variant the wall time to tolerance, Newton and
+    Krylov iterations, set-ups and the final energy; the FP32 runs' relative energy
+    differences; ``speedup`` (fp64 / fp32: precision alone) and, with the reuse variants,
+    ``speedup_mixed_form`` (fp64 / fp32_reuse) and ``speedup_policy`` (fp64 / fp64_reuse)."""
     from .drivers import solve_newton
     from .problems.ginzburg_landau import GLReorderedElementAssembler, build_gl_problem, sine_noise_init
 
@@ -305,25 +311,34 @@ def gl_precision_study(*, n=1024, seed=0, ksp_type="stcg", pc_type="gamg", ksp_r
     params, adj = build_gl_problem(n)
     u0 = sine_noise_init(params, seed)
     out = {"n_elem": int(params["elems"].shape[0]), "n_free": int(params["freedofs"].size), "variants": {}}
-    for prec in ("fp64", "fp32"):
+    variants = [("fp64", "fp64", False), ("fp32", "fp32", False)]
+    if pc_refresh_its is not None:
+        variants += [("fp64_reuse", "fp64", True), ("fp32_reuse", "fp32", True)]
+    for name, prec, reuse in variants:
         asm = GLReorderedElementAssembler(params, None, adj, ksp_type=ksp_type, pc_type=pc_type, ksp_rtol=ksp_rtol,
                                           ksp_max_it=ksp_max_it, hessian_precision=prec,
                                           pc_options={"mg_coarse_factor": coarse_factor})
         x = asm.create_vec(u0[asm.layout.perm])
+        policy = dict(pc_setup_on_ksp_cap=True, keep_pc=True, pc_refresh_its=pc_refresh_its) if reuse else {}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = solve_newton(asm, x, ksp_rtol=ksp_rtol, ksp_max_it=ksp_max_it, use_trust_region=True,
                            trust_radius_init=trust_radius_init, trust_subproblem_line_search=True,
-                           linesearch_tol=1e-3, **nk)
+                           linesearch_tol=1e-3, **policy, **nk)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         amg = getattr(asm.ksp, "_amg", None)
-        out["variants"][prec] = {"wall_s": wall, "newton_its": int(res["nit"]), "linear_its": int(res["linear_iters"]),
+        out["variants"][name] = {"wall_s": wall, "newton_its": int(res["nit"]), "linear_its": int(res["linear_iters"]),
                                  "message": res["message"], "energy": float(res["fun"]),
                                  "pc_setups": int(getattr(amg, "setup_count", 0)) if amg is not None else 0}
-        log(f"{prec}: {res['message']} nit={res['nit']} lin={res['linear_iters']} E={res['fun']!r} wall={wall:.2f}s")
+        log(f"{name}: {res['message']} nit={res['nit']} lin={res['linear_iters']} E={res['fun']!r} wall={wall:.2f}s")
         asm.cleanup()
-    v64, v32 = out["variants"]["fp64"], out["variants"]["fp32"]
-    out["energy_rel_diff"] = abs(v32["energy"] - v64["energy"]) / abs(v64["energy"])
-    out["speedup"] = v64["wall_s"] / v32["wall_s"]
+    v = out["variants"]
+    e64 = v["fp64"]["energy"]
+    out["energy_rel_diff"] = abs(v["fp32"]["energy"] - e64) / abs(e64)
+    out["speedup"] = v["fp64"]["wall_s"] / v["fp32"]["wall_s"]
+    if "fp32_reuse" in v:
+        out["energy_rel_diff_mixed_form"] = abs(v["fp32_reuse"]["energy"] - e64) / abs(e64)
+        out["speedup_mixed_form"] = v["fp64"]["wall_s"] / v["fp32_reuse"]["wall_s"]
+        out["speedup_policy"] = v["fp64"]["wall_s"] / v["fp64_reuse"]["wall_s"]
     return out

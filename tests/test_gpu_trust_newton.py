@@ -134,8 +134,9 @@ def test_gl_sine_noise_trust_region_fp32_hessian_reaches_the_fp64_energy(require
 def test_gl_precision_study_both_precisions_converge_to_the_same_energy(require_gpu):
     """The headline configuration end to end at 64 x 64 cells (the function behind the bench
     line's ``time_to_tol_headline``): trust-region Newton with Steihaug CG and the multigrid
-    converges with FP64 and with FP32 Hessian values and Krylov products, and the FP32 run's
-    final energy is within the stated mixed-precision bound of the FP64 run's."""
+    converges with FP64 and with FP32 Hessian values and Krylov products, with the multigrid set
+    up at every Newton iteration and with it kept across iterations, and every run's final
+    energy is within the stated mixed-precision bound of the FP64 run's."""
     from fenics_nonlinear_energies_amd.experiments import gl_precision_study
 
     out = gl_precision_study(n=64, seed=0, log=lambda *a: None)
@@ -145,4 +146,11 @@ def test_gl_precision_study_both_precisions_converge_to_the_same_energy(require_
     assert out["n_free"] == 63 * 63
     assert "converged" in v64["message"].lower() and "converged" in v32["message"].lower()
     assert out["energy_rel_diff"] <= MIXED_REL
-    assert v64["pc_setups"] >= 1 and v32["pc_setups"] >= 1
+    assert v64["pc_setups"] >= v64["newton_its"] and v32["pc_setups"] >= v32["newton_its"]
+    # the multigrid kept across Newton iterations: no more set-ups, the same end point
+    for name, every in (("fp64_reuse", v64), ("fp32_reuse", v32)):
+        r = out["variants"][name]
+        print(f"GL 64^2 {name}: nit={r['newton_its']} set-ups={r['pc_setups']} E={r['energy']!r}", flush=True)
+        assert "converged" in r["message"].lower()
+        assert 1 <= r["pc_setups"] <= every["pc_setups"]
+        assert abs(r["energy"] - v64["energy"]) <= MIXED_REL * abs(v64["energy"])
