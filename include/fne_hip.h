@@ -91,7 +91,7 @@ typedef struct fne_plan_desc {
   double param[8];        /* p | eps | C1,D1 | E,nu,phi_deg,cohesion,reg */
   const double* qdata;    /* quadrature / tabulation table (GL: ip[3][nq] then w[nq]; with
                              FNE_FLAG_GL_PRODUCTS then w[q] ip[a][q] ip[b][q] for ab = 00 11 22 01 02 12,
-                             [6][nq]) */
+                             [6][nq]: 10 nq doubles in all, which the caller must provide) */
   int32_t nq;             /* quadrature points */
   int32_t flags;          /* FNE_FLAG_* */
   /* P1 with FNE_FLAG_NODE_GEO: vertex coordinates (x, y) of every DOF code, [n_free + n_total][2]
